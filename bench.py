@@ -1,0 +1,221 @@
+"""Benchmark: Newton solve wall time and assembly throughput, mixed FP32/FP64 vs
+all-FP64, on one MI355X, at relative residual <= 1e-8.
+
+Workload: the NeoHookean compression cube -- 1 x 1 x 1 m, 64^3 hexahedra x 6
+tets (1,572,864 tets, 274,625 nodes, 823,875 dofs), mu = 3.8e5 Pa, K = 8.3e5 Pa,
+symmetry rollers on x=0 / y=0 / z=0, compressive pressure on x=1 ramped over 20
+load steps to 15 % nominal strain, +-2 % interior-node jitter (seed 2).  A
+"step" is one full NonLinearStaticAnalysis solve (20 load steps) with
+NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-14, rel_res_force_tol=1e-8,
+max_iter=30)).  Inputs are resident in HBM before the timed region (the mesh,
+colouring and block pattern are built once, outside it).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-fp64] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "onsas.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_mode(s, sm, p, precision, steps, warmup, nsteps, rtol=None):
+    from onsas_amd.analysis import (ConvergenceSettings, NewtonRaphson, NonLinearStaticAnalysis,
+                                    ResidualsIterationStep, solve_)
+    from onsas_amd.backend import GpuStaticState, MI355XBackend
+    from onsas_amd import workloads
+
+    be = MI355XBackend(precision=precision, rtol=rtol, maxiter=20000, store_stresses=False)
+    t0 = time.perf_counter()
+    state = GpuStaticState(s, be)
+    setup_s = time.perf_counter() - t0
+    alg = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-14, rel_res_force_tol=1e-8, max_iter=30))
+    sa = NonLinearStaticAnalysis(s, NSTEPS=nsteps)
+    zeros = np.zeros(s.num_dofs())
+    walls, sols = [], []
+    for it in range(warmup + steps):
+        sa.current_step = 1
+        sa.iter_state = ResidualsIterationStep()
+        sa.state = state
+        state.ctx.set_U(zeros)
+        if it == warmup:
+            state.ctx.reset_timings()
+        t0 = time.perf_counter()
+        sol = solve_(sa, alg, be)
+        dt = time.perf_counter() - t0
+        log(f"{precision} solve {it} ({'warmup' if it < warmup else 'timed'}): {dt:.3f} s, "
+            f"newton {sum(sol.iterations())}, pcg {sol.linear_iterations()}")
+        if it >= warmup:
+            walls.append(dt)
+            sols.append(sol)
+    tm = state.ctx.timings()
+    nn, nnzb, ncol = state.ctx.sizes()
+    sol = sols[-1]
+    a = workloads.nh_uniaxial_alpha(-p, 8.3e5, 3.8e5)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    ex = workloads.homogeneous_displacement(sm.coords, a, b)
+    err = float(np.abs(sol.U[-1] - ex).max() / np.abs(ex).max())
+    out = dict(
+        wall_s=float(np.mean(walls)), walls=[round(w, 4) for w in walls], setup_s=setup_s,
+        newton_iters=int(sum(sol.iterations())), pcg_iters=int(sol.linear_iterations()),
+        max_rel_residual=float(max(r["dr_rel"] for r in sol.records)),
+        criteria=sorted({type(r["criterion"]).__name__ for r in sol.records}),
+        disp_err_vs_closed_form=err, timings=tm, nn=nn, nnzb=nnzb, ncolors=ncol,
+        U_final=sol.U[-1])
+    state.close()
+    return out
+
+
+def cpu_baseline(n_small=16, nsteps=20, n_full=64):
+    """The oracle (reference algorithm: COO->CSR assembly, unpreconditioned CG at
+    reltol sqrt(eps)) on one core, full solve of a smaller cube, projected to the
+    full cube with the (n_full / n_small)^4 work growth of unpreconditioned CG."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from onsas_amd import workloads
+
+    s, sm, p = workloads.cube_compression(n=n_small)
+    (mat, es), = list(s.materials)
+    m = O.Model(sm.coords, tets=es.conn, tet_mat=O.MAT_NEOHOOKEAN, tet_p=mat.params(),
+                fixed_dofs=s.fixed_dofs)
+    lfs = np.linspace(1 / nsteps, 1.0, nsteps)
+    F = []
+    for t in lfs:
+        f = np.zeros(s.num_dofs())
+        for bc, conn in s.bcs.pressure_groups():
+            for tri in conn:
+                fe = O.face_pressure(sm.coords[tri].reshape(-1), bc(t))
+                for a in range(3):
+                    f[3 * tri[a]:3 * tri[a] + 3] += fe[a]
+        F.append(f)
+    F = np.array(F)
+    t0 = time.perf_counter()
+    _, _, recs = m.newton(F, rel_U_tol=1e-14, rel_res_tol=1e-8, max_iter=30, keep_steps=False)
+    dt = time.perf_counter() - t0
+    # assembly rate of the reference algorithm on the same cube
+    U = np.zeros(s.num_dofs())
+    t1 = time.perf_counter()
+    m.assemble(U, want_full=False, want_ff=True)
+    ta = time.perf_counter() - t1
+    scale = (n_full / n_small) ** 4
+    return dict(
+        value=dt * scale, unit="s", cores=1, kind="port",
+        sample=(f"full {nsteps}-step NewtonRaphson solve of the {n_small}^3 x 6-tet NeoHookean "
+                f"compression cube ({len(sm.tets)} tets) with the oracle's restatement of the "
+                f"reference algorithm on 1 core: {dt:.2f} s, {sum(r['iters'] for r in recs)} Newton "
+                f"iterations, {sum(r['cg_iters'] for r in recs)} CG iterations; projected to "
+                f"{n_full}^3 by (n_full/n_small)^4 = {scale:.0f} (unpreconditioned CG: iterations "
+                f"~ n, work per iteration ~ n^3); extrapolated, not measured at {n_full}^3"),
+        measured_s=dt, assembly_Melem_per_s=len(sm.tets) / ta / 1e6)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--nsteps", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this engine has no multi-GPU path); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from onsas_amd import workloads
+    log(f"building the {args.n}^3 x 6-tet compression cube")
+    s, sm, p = workloads.cube_compression(n=args.n)
+    ne = len(sm.tets)
+
+    mixed = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps)
+    fp64 = None if args.no_fp64 else run_mode(s, sm, p, "fp64", args.steps, args.warmup,
+                                              args.nsteps)
+
+    def asm_rates(r, kt):
+        tm = r["timings"]
+        na = max(tm["n_assemble"], 1)
+        scatter_ms = tm["scatter_ms"] / na
+        asm_ms = tm["assemble_ms"] / na
+        bytes_elem = 16 + 104 + 96 + 64 + 2 * 16 * 9 * kt + 2 * 12 * 8
+        return dict(assemble_ms=asm_ms, scatter_ms=scatter_ms,
+                    assembly_Melem_per_s=ne / (asm_ms * 1e-3) / 1e6,
+                    scatter_Melem_per_s=ne / (scatter_ms * 1e-3) / 1e6,
+                    scatter_GBps_algorithmic=ne * bytes_elem / (scatter_ms * 1e-3) / 1e9,
+                    bytes_per_elem=bytes_elem)
+
+    def spmv_roofline(r, kt):
+        tm = r["timings"]
+        nn, nnzb = r["nn"], r["nnzb"]
+        n = 3 * nn
+        # pcg_A: K blocks + col + row_ptr + gathered z and p_old + written p_new and q
+        bytes_launch = nnzb * (9 * kt + 4) + 4 * (nn + 1) + 4 * n * kt
+        avg_ms = tm["spmv_ms"] / max(tm["spmv_launches"], 1)
+        ach = bytes_launch / (avg_ms * 1e-3) / 1e9
+        it = max(tm["pcg_iters"], 1)
+        return dict(bytes_per_launch=bytes_launch, avg_launch_ms=avg_ms, achieved_GBps=ach,
+                    pcg_ms_per_iter=tm["pcg_ms"] / it)
+
+    m_asm = asm_rates(mixed, 4)
+    m_spmv = spmv_roofline(mixed, 4)
+    result = {
+        "metric": "Newton solve wall time & assembly Melem/s, mixed vs FP64 on 1 GPU at resid<=1e-8",
+        "value": round(mixed["wall_s"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["wall_s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"NeoHookean compression cube {args.n}^3x6 tets ({ne} tets, "
+                               f"{3 * sm.nn} dofs), {args.nsteps} load steps to 15% strain, "
+                               f"mixed FP32 tangent/PCG + FP64 residual/update",
+                   "tets": ne, "dofs": 3 * sm.nn, "load_steps": args.nsteps,
+                   "rel_res_force_tol": 1e-8},
+        "roofline": {"bound": "hbm", "achieved": round(m_spmv["achieved_GBps"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(m_spmv["achieved_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pcg_A (BSR 3x3 SpMV + p update, FP32)",
+                     "bytes_per_launch": m_spmv["bytes_per_launch"],
+                     "avg_launch_ms": round(m_spmv["avg_launch_ms"], 5)},
+        "mixed": {k: v for k, v in mixed.items() if k not in ("U_final", "timings")},
+        "mixed_assembly": m_asm,
+        "mixed_pcg": m_spmv,
+    }
+    result["mixed"]["timings"] = mixed["timings"]
+    if fp64 is not None:
+        f_asm = asm_rates(fp64, 8)
+        f_spmv = spmv_roofline(fp64, 8)
+        result["fp64"] = {k: v for k, v in fp64.items() if k not in ("U_final", "timings")}
+        result["fp64"]["timings"] = fp64["timings"]
+        result["fp64_assembly"] = f_asm
+        result["fp64_pcg"] = f_spmv
+        result["speedup_mixed_vs_fp64"] = round(fp64["wall_s"] / mixed["wall_s"], 3)
+        d = np.abs(mixed["U_final"] - fp64["U_final"]).max() / np.abs(fp64["U_final"]).max()
+        result["mixed_vs_fp64_max_rel_disp_diff"] = float(d)
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 core)")
+        result["cpu_baseline"] = cpu_baseline(nsteps=args.nsteps, n_full=args.n)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

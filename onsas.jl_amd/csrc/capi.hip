@@ -1,0 +1,549 @@
+// capi.hip -- the extern "C" boundary of libonsas_hip (declared in
+// include/onsas_hip.h).  Argument checking, ownership and the orchestration of
+// one Newton iteration; the numerics live in element_kernels.hip and
+// solver_kernels.hip, the setup in setup.hip.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+namespace onsas {
+
+int set_err(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+}  // namespace onsas
+
+using onsas::Ctx;
+using onsas::ElemGroup;
+using onsas::set_err;
+
+namespace {
+
+std::string g_create_err;
+
+Ctx* C(onsas_ctx* ctx) { return ctx ? &ctx->impl : nullptr; }
+
+int check_ready(Ctx* c) {
+  if (!c) return ONSAS_ERR_ARG;
+  if (c->injected) {
+    int code = c->injected;
+    c->injected = 0;
+    return set_err(c, code, "injected error");
+  }
+  if (!c->finalized) return set_err(c, ONSAS_ERR_STATE, "onsas_finalize was not called");
+  return ONSAS_OK;
+}
+
+int check_building(Ctx* c) {
+  if (!c) return ONSAS_ERR_ARG;
+  if (c->injected) {
+    int code = c->injected;
+    c->injected = 0;
+    return set_err(c, code, "injected error");
+  }
+  if (c->finalized) return set_err(c, ONSAS_ERR_STATE, "model already finalized");
+  return ONSAS_OK;
+}
+
+// scratch device buffer for host<->device transfers of n doubles
+struct DevScratch {
+  double* p = nullptr;
+  ~DevScratch() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int to_device(Ctx* c, DevScratch& s, const double* h, size_t count) {
+  ONSAS_HIP_TRY(c, hipMalloc((void**)&s.p, std::max<size_t>(count, 1) * sizeof(double)));
+  if (h && count)
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(s.p, h, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return ONSAS_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// Solve K x = b_dev with the configured linear solver; x left in the PCG x buffer (KT).
+int linear_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
+                 int* solver) {
+  int it = 0, conv = 0, brk = 0;
+  double rr = 1.0;
+  *solver = 0;
+  *iters = 0;
+  const int kind = c->lin.kind;
+  if (kind == ONSAS_LINSOLVE_PCG || kind == ONSAS_LINSOLVE_AUTO) {
+    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, &it, &rr, &conv, &brk));
+    *solver = 1;
+    *iters = it;
+    c->tm.pcg_iters += it;
+    if (brk && kind == ONSAS_LINSOLVE_PCG)
+      return set_err(c, ONSAS_ERR_LINEAR_SOLVE,
+                     "PCG breakdown (p'Kp <= 0): the tangent is not positive definite");
+    if (!brk) {
+      *relres = rr;
+      *converged = conv;
+      return ONSAS_OK;
+    }
+  }
+  int git = 0;
+  ONSAS_TRY(onsas::launch_gmres_solve(c, b_dev, &git, &rr, &conv));
+  *solver = (*solver == 1) ? 3 : 2;
+  *iters += git;
+  c->tm.gmres_iters += git;
+  *relres = rr;
+  *converged = conv;
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* onsas_version(void) { return "onsas_hip 0.1.0 (gfx950)"; }
+
+int onsas_create(int32_t precision, onsas_ctx** out) {
+  if (!out) return ONSAS_ERR_ARG;
+  *out = nullptr;
+  if (precision != ONSAS_PRECISION_MIXED && precision != ONSAS_PRECISION_FP64) {
+    g_create_err = "precision must be ONSAS_PRECISION_MIXED or ONSAS_PRECISION_FP64";
+    return ONSAS_ERR_ARG;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    g_create_err = std::string("no HIP device available: ") + hipGetErrorString(e);
+    return ONSAS_ERR_HIP;
+  }
+  onsas_ctx* ctx = new (std::nothrow) onsas_ctx();
+  if (!ctx) return ONSAS_ERR_ALLOC;
+  Ctx* c = &ctx->impl;
+  c->precision = precision;
+  (void)hipGetDevice(&c->device);
+  e = hipStreamCreate(&c->stream);  // default priority
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete ctx;
+    return ONSAS_ERR_HIP;
+  }
+  c->lin.kind = ONSAS_LINSOLVE_AUTO;
+  c->lin.maxiter = 0;
+  c->lin.rtol = 0;
+  c->lin.gmres_restart = 40;
+  c->lin.check_every = 16;
+  *out = ctx;
+  return ONSAS_OK;
+}
+
+void onsas_destroy(onsas_ctx* ctx) {
+  if (!ctx) return;
+  Ctx* c = &ctx->impl;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
+  for (auto& g : c->groups)
+    for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm})
+      if (p) (void)hipFree(p);
+  void* ptrs[] = {c->d_x, c->d_y, c->d_z, c->d_fixmask, c->d_row_ptr, c->d_col, c->d_diag, c->d_K,
+                  c->d_Dinv, c->d_U, c->d_Uasm, c->d_fint, c->d_fext, c->d_res, c->d_nodal, c->d_px, c->d_pr,
+                  c->d_pz, c->d_pp0, c->d_pp1, c->d_pq, c->d_part, c->d_sc, c->d_face_conn,
+                  c->d_face_grp, c->d_nf_ptr, c->d_nf_idx, c->d_pressures, c->d_gm_V, c->d_gm_w,
+                  c->d_gm_h};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_sc) (void)hipHostFree(c->h_sc);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->ev_spmv)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete ctx;
+}
+
+const char* onsas_last_error(const onsas_ctx* ctx) {
+  return ctx ? ctx->impl.err.c_str() : g_create_err.c_str();
+}
+
+int onsas_set_nodes(onsas_ctx* ctx, int64_t nn, const double* x, const double* y, const double* z) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  if (nn <= 0 || !x || !y || !z) return set_err(c, ONSAS_ERR_ARG, "onsas_set_nodes: bad arguments");
+  c->nn = nn;
+  c->X.assign(x, x + nn);
+  c->Y.assign(y, y + nn);
+  c->Z.assign(z, z + nn);
+  return ONSAS_OK;
+}
+
+int onsas_add_tet_group(onsas_ctx* ctx, int64_t ne, const int32_t* conn, int32_t material,
+                        const double* params, const double* shear_per_elem, int32_t* group) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  if (c->nn <= 0) return set_err(c, ONSAS_ERR_STATE, "call onsas_set_nodes first");
+  if (ne <= 0 || !conn || !params)
+    return set_err(c, ONSAS_ERR_ARG, "onsas_add_tet_group: bad arguments");
+  if (material != ONSAS_MAT_SVK && material != ONSAS_MAT_NEOHOOKEAN && material != ONSAS_MAT_LINEAR)
+    return set_err(c, ONSAS_ERR_ARG, "onsas_add_tet_group: unknown material");
+  if (shear_per_elem && material == ONSAS_MAT_LINEAR)
+    return set_err(c, ONSAS_ERR_UNSUPPORTED, "per-element shear modulus needs SVK or NeoHookean");
+  for (int64_t k = 0; k < 4 * ne; ++k)
+    if (conn[k] < 0 || conn[k] >= c->nn)
+      return set_err(c, ONSAS_ERR_ARG, "onsas_add_tet_group: node id out of range");
+  ElemGroup g;
+  g.type = onsas::kTet;
+  g.nnode = 4;
+  g.mat = material;
+  g.p0 = params[0];
+  g.p1 = params[1];
+  g.ne = ne;
+  g.conn.assign(conn, conn + 4 * ne);
+  if (shear_per_elem) {
+    g.per_elem_G = true;
+    g.Gel.assign(shear_per_elem, shear_per_elem + ne);
+  }
+  c->groups.push_back(std::move(g));
+  if (group) *group = (int32_t)c->groups.size() - 1;
+  return ONSAS_OK;
+}
+
+int onsas_add_truss_group(onsas_ctx* ctx, int64_t ne, const int32_t* conn, int32_t strain_model,
+                          double E, const double* area, int32_t* group) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  if (c->nn <= 0) return set_err(c, ONSAS_ERR_STATE, "call onsas_set_nodes first");
+  if (ne <= 0 || !conn || !area) return set_err(c, ONSAS_ERR_ARG, "onsas_add_truss_group: bad arguments");
+  if (strain_model != ONSAS_STRAIN_ROTATED && strain_model != ONSAS_STRAIN_GREEN)
+    return set_err(c, ONSAS_ERR_ARG, "onsas_add_truss_group: unknown strain model");
+  for (int64_t k = 0; k < 2 * ne; ++k)
+    if (conn[k] < 0 || conn[k] >= c->nn)
+      return set_err(c, ONSAS_ERR_ARG, "onsas_add_truss_group: node id out of range");
+  ElemGroup g;
+  g.type = onsas::kTruss;
+  g.nnode = 2;
+  g.mat = strain_model;
+  g.p0 = E;
+  g.ne = ne;
+  g.conn.assign(conn, conn + 2 * ne);
+  g.area.assign(area, area + ne);
+  c->groups.push_back(std::move(g));
+  if (group) *group = (int32_t)c->groups.size() - 1;
+  return ONSAS_OK;
+}
+
+int onsas_add_pressure_faces(onsas_ctx* ctx, int64_t nf, const int32_t* conn, int32_t* group) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  if (c->nn <= 0) return set_err(c, ONSAS_ERR_STATE, "call onsas_set_nodes first");
+  if (nf <= 0 || !conn) return set_err(c, ONSAS_ERR_ARG, "onsas_add_pressure_faces: bad arguments");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (conn[k] < 0 || conn[k] >= c->nn)
+      return set_err(c, ONSAS_ERR_ARG, "onsas_add_pressure_faces: node id out of range");
+  c->pressure_faces.emplace_back(conn, conn + 3 * nf);
+  if (group) *group = (int32_t)c->pressure_faces.size() - 1;
+  return ONSAS_OK;
+}
+
+int onsas_set_fixed_dofs(onsas_ctx* ctx, int64_t nfixed, const int64_t* dofs) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  if (c->nn <= 0) return set_err(c, ONSAS_ERR_STATE, "call onsas_set_nodes first");
+  if (nfixed < 0 || (nfixed > 0 && !dofs)) return set_err(c, ONSAS_ERR_ARG, "bad fixed dofs");
+  for (int64_t k = 0; k < nfixed; ++k)
+    if (dofs[k] < 0 || dofs[k] >= 3 * c->nn) return set_err(c, ONSAS_ERR_ARG, "fixed dof out of range");
+  c->fixed.assign(dofs, dofs + nfixed);
+  return ONSAS_OK;
+}
+
+int onsas_finalize(onsas_ctx* ctx) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_building(c));
+  ONSAS_HIP_TRY(c, hipSetDevice(c->device));
+  return onsas::finalize_model(c);
+}
+
+int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s) {
+  Ctx* c = C(ctx);
+  if (!c || !s) return ONSAS_ERR_ARG;
+  if (s->kind < ONSAS_LINSOLVE_AUTO || s->kind > ONSAS_LINSOLVE_GMRES)
+    return set_err(c, ONSAS_ERR_ARG, "unknown linear solver kind");
+  c->lin = *s;
+  if (c->lin.gmres_restart <= 0) c->lin.gmres_restart = 40;
+  if (c->lin.check_every <= 0) c->lin.check_every = 16;
+  return ONSAS_OK;
+}
+
+int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, const double* pressures) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (npressure != (int32_t)c->pressure_faces.size() || (npressure > 0 && !pressures))
+    return set_err(c, ONSAS_ERR_ARG, "onsas_set_loads: one pressure value per pressure group expected");
+  const size_t n = 3 * (size_t)c->nn;
+  if (nodal)
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_nodal, nodal, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  else
+    ONSAS_HIP_TRY(c, hipMemsetAsync(c->d_nodal, 0, n * sizeof(double), c->stream));
+  if (npressure > 0)
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_pressures, pressures, npressure * sizeof(double),
+                                    hipMemcpyHostToDevice, c->stream));
+  ONSAS_TRY(onsas::launch_fext(c));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_assemble(onsas_ctx* ctx, double* fint_out) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_Uasm, c->d_U, 3 * c->nn * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+  ONSAS_TRY(onsas::launch_assemble(c));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  if (fint_out)
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(fint_out, c->d_fint, 3 * c->nn * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->last_assemble_ms = elapsed(c->ev[0], c->ev[3]);
+  c->tm.assemble_ms += c->last_assemble_ms;
+  c->tm.scatter_ms += elapsed(c->ev[1], c->ev[2]);
+  c->tm.n_assemble++;
+  c->assembled = true;
+  return ONSAS_OK;
+}
+
+int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!out) return set_err(c, ONSAS_ERR_ARG, "onsas_newton_step: NULL stats");
+  if (!c->assembled) return set_err(c, ONSAS_ERR_STATE, "onsas_assemble must precede onsas_newton_step");
+  std::memset(out, 0, sizeof *out);
+  hipStream_t s = c->stream;
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[4], s));
+  ONSAS_TRY(onsas::launch_residual(c));
+  int iters = 0, conv = 0, solver = 0;
+  double relres = 0;
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[5], s));
+  ONSAS_TRY(linear_solve(c, c->d_res, &iters, &relres, &conv, &solver));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[6], s));
+  ONSAS_TRY(onsas::launch_update(c));
+  ONSAS_TRY(onsas::launch_finalize_newton(c));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[7], s));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(&c->h_sc->nwt, &c->d_sc->nwt, sizeof(onsas::NewtonStats),
+                                  hipMemcpyDeviceToHost, s));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
+  const onsas::NewtonStats& nw = c->h_sc->nwt;
+  out->r_norm = std::sqrt(nw.r2);
+  out->fext_norm = std::sqrt(nw.fext2);
+  out->dU_norm = std::sqrt(nw.dU2);
+  out->U_norm = std::sqrt(nw.U2);
+  out->r_rel = out->r_norm / out->fext_norm;
+  out->dU_rel = out->dU_norm / out->U_norm;
+  out->lin_iters = iters;
+  out->lin_solver = solver;
+  out->lin_converged = conv;
+  out->lin_rel_residual = relres;
+  out->assemble_ms = c->last_assemble_ms;
+  out->solve_ms = elapsed(c->ev[4], c->ev[7]);
+  c->tm.solve_ms += out->solve_ms;
+  c->tm.pcg_ms += elapsed(c->ev[5], c->ev[6]);
+  c->tm.n_steps++;
+  c->assembled = false;  // the tangent belongs to the previous U
+  if (!std::isfinite(out->r_norm) || !std::isfinite(out->dU_norm))
+    return set_err(c, ONSAS_ERR_NOT_FINITE, "non-finite residual or update norm");
+  return ONSAS_OK;
+}
+
+int onsas_get_U(onsas_ctx* ctx, double* U) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(U, c->d_U, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_set_U(onsas_ctx* ctx, const double* U) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL input");
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_U, U, 3 * c->nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->assembled = false;
+  return ONSAS_OK;
+}
+
+int onsas_get_fext(onsas_ctx* ctx, double* f) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!f) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(f, c->d_fext, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_get_fint(onsas_ctx* ctx, double* f) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!f) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(f, c->d_fint, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_get_sizes(onsas_ctx* ctx, int64_t* nn, int64_t* nnzb, int64_t* ncolors) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (nn) *nn = c->nn;
+  if (nnzb) *nnzb = c->nnzb;
+  if (ncolors) {
+    int64_t k = 0;
+    for (auto& g : c->groups) k += (int64_t)g.color_off.size() - 1;
+    *ncolors = k;
+  }
+  return ONSAS_OK;
+}
+
+int onsas_get_pattern(onsas_ctx* ctx, int32_t* row_ptr, int32_t* col) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (row_ptr) std::memcpy(row_ptr, c->row_ptr.data(), (c->nn + 1) * sizeof(int32_t));
+  if (col) std::memcpy(col, c->col.data(), c->nnzb * sizeof(int32_t));
+  return ONSAS_OK;
+}
+
+int onsas_get_tangent(onsas_ctx* ctx, double* vals) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!vals) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  DevScratch tmp;
+  ONSAS_TRY(to_device(c, tmp, nullptr, 9 * c->nnzb));
+  ONSAS_TRY(onsas::launch_convert_K(c, tmp.p));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(vals, tmp.p, 9 * c->nnzb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!x || !y) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
+  const size_t n = 3 * (size_t)c->nn;
+  DevScratch dx, dy;
+  ONSAS_TRY(to_device(c, dx, x, n));
+  ONSAS_TRY(to_device(c, dy, nullptr, n));
+  ONSAS_TRY(onsas::launch_matvec(c, dx.p, dy.p));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+namespace {
+int element_outputs(Ctx* c, int32_t group, const double* U, double* f, double* K, double* stress,
+                    double* strain) {
+  ONSAS_TRY(check_ready(c));
+  if (group < 0 || group >= (int32_t)c->groups.size()) return set_err(c, ONSAS_ERR_ARG, "bad group");
+  const ElemGroup& g = c->groups[group];
+  const int nd = 3 * g.nnode;
+  const size_t ne = g.ne;
+  DevScratch df, dK, ds, de;
+  if (f) ONSAS_TRY(to_device(c, df, nullptr, ne * nd));
+  if (K) ONSAS_TRY(to_device(c, dK, nullptr, ne * nd * nd));
+  if (stress) ONSAS_TRY(to_device(c, ds, nullptr, ne * 9));
+  if (strain) ONSAS_TRY(to_device(c, de, nullptr, ne * 9));
+  ONSAS_TRY(onsas::launch_element_outputs(c, group, U, df.p, dK.p, ds.p, de.p));
+  if (f) ONSAS_HIP_TRY(c, hipMemcpyAsync(f, df.p, ne * nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (K) ONSAS_HIP_TRY(c, hipMemcpyAsync(K, dK.p, ne * nd * nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (stress) ONSAS_HIP_TRY(c, hipMemcpyAsync(stress, ds.p, ne * 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (strain) ONSAS_HIP_TRY(c, hipMemcpyAsync(strain, de.p, ne * 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+int onsas_element_forces(onsas_ctx* ctx, int32_t group, double* f, double* K, double* stress,
+                         double* strain) {
+  Ctx* c = C(ctx);
+  return element_outputs(c, group, c ? c->d_U : nullptr, f, K, stress, strain);
+}
+
+int onsas_get_stress_strain(onsas_ctx* ctx, int32_t group, int32_t at_last_assembly,
+                            double* stress, double* strain) {
+  Ctx* c = C(ctx);
+  if (c && at_last_assembly && c->tm.n_assemble == 0)
+    return set_err(c, ONSAS_ERR_STATE, "no assembly has been done yet");
+  return element_outputs(c, group, c ? (at_last_assembly ? c->d_Uasm : c->d_U) : nullptr, nullptr,
+                         nullptr, stress, strain);
+}
+
+int onsas_linear_solve(onsas_ctx* ctx, const double* b, double* x, int32_t* iters, double* rel_residual) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!b || !x) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
+  if (!c->assembled && c->tm.n_assemble == 0) return set_err(c, ONSAS_ERR_STATE, "assemble first");
+  const size_t n = 3 * (size_t)c->nn;
+  DevScratch db, dx;
+  ONSAS_TRY(to_device(c, db, b, n));
+  ONSAS_TRY(to_device(c, dx, nullptr, n));
+  int it = 0, conv = 0, solver = 0;
+  double rr = 0;
+  ONSAS_TRY(linear_solve(c, db.p, &it, &rr, &conv, &solver));
+  ONSAS_TRY(onsas::launch_x_to_double(c, dx.p));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (iters) *iters = it;
+  if (rel_residual) *rel_residual = rr;
+  if (!conv) return set_err(c, ONSAS_ERR_LINEAR_SOLVE, "linear solver did not reach its tolerance");
+  return ONSAS_OK;
+}
+
+int onsas_adjoint_shear(onsas_ctx* ctx, int32_t group, const double* dJdU, double* dJdG, int32_t* iters) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (group < 0 || group >= (int32_t)c->groups.size()) return set_err(c, ONSAS_ERR_ARG, "bad group");
+  const ElemGroup& g = c->groups[group];
+  if (g.type != onsas::kTet || (g.mat != ONSAS_MAT_NEOHOOKEAN && g.mat != ONSAS_MAT_SVK))
+    return set_err(c, ONSAS_ERR_UNSUPPORTED, "adjoint shear sensitivity needs an SVK or NeoHookean tet group");
+  if (!dJdU || !dJdG) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
+  if (!c->assembled && c->tm.n_assemble == 0) return set_err(c, ONSAS_ERR_STATE, "assemble first");
+  const size_t n = 3 * (size_t)c->nn;
+  DevScratch db, dout;
+  ONSAS_TRY(to_device(c, db, dJdU, n));
+  ONSAS_TRY(to_device(c, dout, nullptr, g.ne));
+  int it = 0, conv = 0, solver = 0;
+  double rr = 0;
+  ONSAS_TRY(linear_solve(c, db.p, &it, &rr, &conv, &solver));
+  if (!conv) return set_err(c, ONSAS_ERR_LINEAR_SOLVE, "adjoint solve did not reach its tolerance");
+  ONSAS_TRY(onsas::launch_adjoint_shear(c, group, c->d_px, dout.p));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(dJdG, dout.p, g.ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (iters) *iters = it;
+  return ONSAS_OK;
+}
+
+int onsas_get_timings(onsas_ctx* ctx, onsas_timings* t) {
+  Ctx* c = C(ctx);
+  if (!c || !t) return ONSAS_ERR_ARG;
+  *t = c->tm;
+  return ONSAS_OK;
+}
+
+int onsas_reset_timings(onsas_ctx* ctx) {
+  Ctx* c = C(ctx);
+  if (!c) return ONSAS_ERR_ARG;
+  c->tm = onsas_timings{};
+  return ONSAS_OK;
+}
+
+int onsas_inject_error(onsas_ctx* ctx, int32_t code) {
+  Ctx* c = C(ctx);
+  if (!c || code >= 0) return ONSAS_ERR_ARG;
+  c->injected = code;
+  return ONSAS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,674 @@
+// element_kernels.hip -- tet4 / truss internal force + tangent, coloured
+// block-CSR scatter, block-Jacobi inverse, triangle-face pressure loads,
+// per-element outputs and the adjoint contraction.  gfx950, wave64.
+//
+// Tangent of a tet in closed form.  With n_a the reference gradient of shape
+// function a and V the reference volume, the reference builds
+//   K = B' dSdE B V + (n' S n V) (x) I3          (src/Entities/Tetrahedrons.jl:207-215)
+// Contracting the Voigt algebra by hand gives, per 3x3 node block (a, b):
+//   K_ab = V [ c1 v_a v_b' + c2 v_b v_a' + c2 s_ab M + sigma_ab I ]
+//   SVK  (SVKMaterial.jl:89-100):          v = F n, s_ab = n_a.n_b, M = F F', c1 = lambda, c2 = G
+//   NeoHookean (NeoHookeanMaterial.jl:94-129): v = F^-T n, s_ab = v_a.v_b, M = I,
+//        c1 = K J (2J - 1), c2 = G - K J (J - 1), J = sqrt(det C)
+//   linear (Tetrahedrons.jl:240-266):      v = n, s_ab = n_a.n_b, M = I, c1 = lambda, c2 = G,
+//        no geometric term
+//   sigma_ab = n_a . S n_b   (geometric stiffness, hyperelastic only)
+// and f_a = V F S n_a  (= B' voigt(S) V restricted to node a).  f and S are
+// always FP64; the block is evaluated in the tangent precision KT.
+//
+// Truss (src/Entities/Trusses.jl:126-184), with d = X2 - X1 + u2 - u1, l = |d|,
+// s_0 = -1, s_1 = +1:
+//   K_ab = s_a s_b [ cI I + cdd d d' ],  f_a = s_a cf d
+//   rotated: eps = (l^2 - l0^2) / (l0 (l0 + l)), S11 = E eps,
+//            cI = S11 A / l, cdd = (E A / l0 - S11 A / l) / l^2, cf = A S11 / l
+//   Green:   eps = (l^2 - l0^2) / (2 l0^2),  S11 = E eps,
+//            cI = S11 A / l0, cdd = E A / l0^3, cf = A S11 / l0
+#include "engine.hpp"
+
+namespace onsas {
+
+namespace {
+
+constexpr int kScatterThreads = 256;
+
+struct TetEval {
+  double F[9];     // deformation gradient (I for linear)
+  double S[9];     // second Piola-Kirchhoff (sigma for linear)
+  double v[4][3];  // per-material gradient vectors (see header comment)
+  double M[9];
+  double c1, c2;
+  bool geo;        // geometric stiffness present
+  bool sv;         // s_ab from v (NeoHookean) instead of n
+};
+
+// p0, p1: SVK (lambda, G), NeoHookean (K, G), linear (E, nu)
+__device__ inline void tet_eval(int mat, double p0, double p1, const double n[4][3],
+                                const double u[4][3], TetEval& t) {
+  double H[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      H[i * 3 + j] = u[0][i] * n[0][j] + u[1][i] * n[1][j] + u[2][i] * n[2][j] + u[3][i] * n[3][j];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    t.F[q] = H[q] + ((q % 4) == 0 ? 1.0 : 0.0);
+    t.M[q] = (q % 4) == 0 ? 1.0 : 0.0;
+  }
+  if (mat == ONSAS_MAT_LINEAR) {
+    // stress! (IsotropicLinearElasticMaterial.jl:80-92), lame_parameters (:70-76)
+    const double lam = p0 * p1 / ((1 + p1) * (1 - 2 * p1));
+    const double G = p0 / (2 * (1 + p1));
+    double tr = H[0] + H[4] + H[8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        t.S[i * 3 + j] = G * (H[i * 3 + j] + H[j * 3 + i]) + (i == j ? lam * tr : 0.0);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) t.F[q] = (q % 4) == 0 ? 1.0 : 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) t.v[b][d] = n[b][d];
+    t.c1 = lam;
+    t.c2 = G;
+    t.geo = false;
+    t.sv = false;
+    return;
+  }
+  const double* F = t.F;
+  if (mat == ONSAS_MAT_SVK) {
+    // E = 1/2 (H + H' + H'H) (Tetrahedrons.jl:201); S = lambda tr(E) I + 2G E
+    const double lam = p0, G = p1;
+    double E[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        E[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i] + H[i] * H[j] + H[3 + i] * H[3 + j] +
+                              H[6 + i] * H[6 + j]);
+    double tr = E[0] + E[4] + E[8];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) t.S[q] = 2 * G * E[q] + ((q % 4) == 0 ? lam * tr : 0.0);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        t.v[b][i] = F[i * 3 + 0] * n[b][0] + F[i * 3 + 1] * n[b][1] + F[i * 3 + 2] * n[b][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        t.M[i * 3 + j] = F[i * 3] * F[j * 3] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
+    t.c1 = lam;
+    t.c2 = G;
+    t.geo = true;
+    t.sv = false;
+    return;
+  }
+  // NeoHookean: S = G (I - C^-1) + K J (J - 1) C^-1, C^-1 = F^-1 F^-T, J = sqrt(det C) = |det F|
+  const double K = p0, G = p1;
+  double cof[9];  // cofactor matrix of F: F^-T = cof / det F
+  cof[0] = F[4] * F[8] - F[5] * F[7];
+  cof[1] = F[5] * F[6] - F[3] * F[8];
+  cof[2] = F[3] * F[7] - F[4] * F[6];
+  cof[3] = F[2] * F[7] - F[1] * F[8];
+  cof[4] = F[0] * F[8] - F[2] * F[6];
+  cof[5] = F[1] * F[6] - F[0] * F[7];
+  cof[6] = F[1] * F[5] - F[2] * F[4];
+  cof[7] = F[2] * F[3] - F[0] * F[5];
+  cof[8] = F[0] * F[4] - F[1] * F[3];
+  const double detF = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2];
+  const double J = fabs(detF);
+  double FiT[9];  // F^-T
+#pragma unroll
+  for (int q = 0; q < 9; ++q) FiT[q] = cof[q] / detF;
+  double Ci[9];  // C^-1 = F^-1 F^-T = (F^-T)' (F^-T)
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Ci[i * 3 + j] = FiT[i] * FiT[j] + FiT[3 + i] * FiT[3 + j] + FiT[6 + i] * FiT[6 + j];
+  const double kjj = K * J * (J - 1);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) t.S[q] = G * (((q % 4) == 0 ? 1.0 : 0.0) - Ci[q]) + kjj * Ci[q];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      t.v[b][i] = FiT[i * 3] * n[b][0] + FiT[i * 3 + 1] * n[b][1] + FiT[i * 3 + 2] * n[b][2];
+  t.c1 = K * J * (2 * J - 1);
+  t.c2 = G - kjj;
+  t.geo = true;
+  t.sv = true;
+}
+
+// f_a = V F S n_a
+__device__ inline void tet_force_row(const TetEval& t, const double n[4][3], double V, int a,
+                                     double fa[3]) {
+  double Sn[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    Sn[i] = t.S[i * 3] * n[a][0] + t.S[i * 3 + 1] * n[a][1] + t.S[i * 3 + 2] * n[a][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    fa[i] = V * (t.F[i * 3] * Sn[0] + t.F[i * 3 + 1] * Sn[1] + t.F[i * 3 + 2] * Sn[2]);
+}
+
+// Element data converted once to the tangent precision.
+template <typename T>
+struct TetBlocks {
+  T n[4][3], v[4][3], M[9], S[9], c1, c2, V;
+  bool geo, sv;
+  __device__ inline void load(const TetEval& t, const double nn[4][3], double Vol) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        n[b][d] = (T)nn[b][d];
+        v[b][d] = (T)t.v[b][d];
+      }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      M[q] = (T)t.M[q];
+      S[q] = (T)t.S[q];
+    }
+    c1 = (T)t.c1;
+    c2 = (T)t.c2;
+    V = (T)Vol;
+    geo = t.geo;
+    sv = t.sv;
+  }
+  __device__ inline void block(int a, int b, T out[9]) const {
+    T s = sv ? (v[a][0] * v[b][0] + v[a][1] * v[b][1] + v[a][2] * v[b][2])
+             : (n[a][0] * n[b][0] + n[a][1] * n[b][1] + n[a][2] * n[b][2]);
+    T sig = 0;
+    if (geo) {
+      T Sn0 = S[0] * n[b][0] + S[1] * n[b][1] + S[2] * n[b][2];
+      T Sn1 = S[3] * n[b][0] + S[4] * n[b][1] + S[5] * n[b][2];
+      T Sn2 = S[6] * n[b][0] + S[7] * n[b][1] + S[8] * n[b][2];
+      sig = n[a][0] * Sn0 + n[a][1] * Sn1 + n[a][2] * Sn2;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        out[i * 3 + j] = V * (c1 * v[a][i] * v[b][j] + c2 * v[b][i] * v[a][j] + c2 * s * M[i * 3 + j] +
+                              (i == j ? sig : T(0)));
+  }
+};
+
+struct TrussEval {
+  double d[3], l, S11, eps, cI, cdd, cf, sigma;
+};
+
+__device__ inline void truss_eval(int strain, double E, double A, const double d0[3], double l0,
+                                  const double du[3], TrussEval& t) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t.d[i] = d0[i] + du[i];
+  t.l = sqrt(t.d[0] * t.d[0] + t.d[1] * t.d[1] + t.d[2] * t.d[2]);
+  // l^2 - l0^2 = 2 d0.du + du.du (no cancellation)
+  const double dl2 = 2 * (d0[0] * du[0] + d0[1] * du[1] + d0[2] * du[2]) +
+                     (du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
+  if (strain == ONSAS_STRAIN_ROTATED) {
+    t.eps = dl2 / (l0 * (l0 + t.l));
+    t.S11 = E * t.eps;
+    t.cI = t.S11 * A / t.l;
+    t.cdd = (E * A / l0 - t.S11 * A / t.l) / (t.l * t.l);
+    t.cf = A * t.S11 / t.l;
+  } else {
+    t.eps = dl2 / (2 * l0 * l0);
+    t.S11 = E * t.eps;
+    t.cI = t.S11 * A / l0;
+    t.cdd = E * A / (l0 * l0 * l0);
+    t.cf = A * t.S11 / l0;
+  }
+  t.sigma = t.S11 * t.l / l0;
+}
+
+struct TetArgs {
+  const int32_t* conn;
+  const double* geo;
+  const int32_t* slot;
+  const double* G;
+  int64_t ne, e0, e1;
+  int mat;
+  double p0, p1;
+};
+
+__device__ inline void tet_load(const TetArgs& ta, int64_t e, const double* __restrict__ U,
+                                int32_t nd[4], double n[4][3], double& V, double u[4][3],
+                                double& p1) {
+  const int64_t ne = ta.ne;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) nd[b] = ta.conn[b * ne + e];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) n[b][d] = ta.geo[(b * 3 + d) * ne + e];
+  V = ta.geo[12 * ne + e];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[b][d] = U[3 * (int64_t)nd[b] + d];
+  p1 = ta.G ? ta.G[e] : ta.p1;
+}
+
+// One lane per (element, row node a): f_a and the four blocks (a, b) of
+// row a, read-modify-written without atomics (the colour makes every node of
+// the launch exclusive to one element).  Entries in a fixed row or column are
+// not added, so those rows/columns keep the identity written by k_fixed_identity.
+template <typename KT>
+__global__ __launch_bounds__(kScatterThreads) void k_tet_scatter(TetArgs ta,
+                                                                  const double* __restrict__ U,
+                                                                  double* __restrict__ fint,
+                                                                  KT* __restrict__ K,
+                                                                  const uint8_t* __restrict__ mask) {
+  const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
+  const int64_t e = ta.e0 + (t >> 2);
+  const int a = (int)(t & 3);
+  if (e >= ta.e1) return;
+  int32_t nd[4];
+  double n[4][3], u[4][3], V, p1;
+  tet_load(ta, e, U, nd, n, V, u, p1);
+  TetEval ev;
+  tet_eval(ta.mat, ta.p0, p1, n, u, ev);
+  double fa[3];
+  tet_force_row(ev, n, V, a, fa);
+  double* fp = fint + 3 * (int64_t)nd[a];
+  fp[0] += fa[0];
+  fp[1] += fa[1];
+  fp[2] += fa[2];
+  TetBlocks<KT> tb;
+  tb.load(ev, n, V);
+  const uint32_t ma = mask[nd[a]];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const uint32_t mb = mask[nd[b]];
+    const int64_t s = ta.slot[(a * 4 + b) * ta.ne + e];
+    KT blk[9];
+    tb.block(a, b, blk);
+    KT* kp = K + 9 * s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const bool off = ((ma >> i) & 1u) | ((mb >> j) & 1u);
+        kp[i * 3 + j] += off ? KT(0) : blk[i * 3 + j];
+      }
+  }
+}
+
+struct TrussArgs {
+  const int32_t* conn;
+  const double* geo;  // d0 (3), l0, A
+  const int32_t* slot;
+  int64_t ne, e0, e1;
+  int strain;
+  double E;
+};
+
+template <typename KT>
+__global__ __launch_bounds__(kScatterThreads) void k_truss_scatter(TrussArgs ta,
+                                                                    const double* __restrict__ U,
+                                                                    double* __restrict__ fint,
+                                                                    KT* __restrict__ K,
+                                                                    const uint8_t* __restrict__ mask) {
+  const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
+  const int64_t e = ta.e0 + (t >> 1);
+  const int a = (int)(t & 1);
+  if (e >= ta.e1) return;
+  const int64_t ne = ta.ne;
+  const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
+  double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
+  const double l0 = ta.geo[3 * ne + e], A = ta.geo[4 * ne + e];
+  double du[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) du[d] = U[3 * (int64_t)nd1 + d] - U[3 * (int64_t)nd0 + d];
+  TrussEval tv;
+  truss_eval(ta.strain, ta.E, A, d0, l0, du, tv);
+  const double sa = a ? 1.0 : -1.0;
+  const int32_t na = a ? nd1 : nd0;
+  double* fp = fint + 3 * (int64_t)na;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fp[d] += sa * tv.cf * tv.d[d];
+  const uint32_t ma = mask[na];
+  const KT cI = (KT)tv.cI, cdd = (KT)tv.cdd;
+  const KT dd[3] = {(KT)tv.d[0], (KT)tv.d[1], (KT)tv.d[2]};
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int32_t nb = b ? nd1 : nd0;
+    const uint32_t mb = mask[nb];
+    const KT sab = (KT)(sa * (b ? 1.0 : -1.0));
+    const int64_t s = ta.slot[(a * 2 + b) * ne + e];
+    KT* kp = K + 9 * s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const bool off = ((ma >> i) & 1u) | ((mb >> j) & 1u);
+        const KT v = sab * ((i == j ? cI : KT(0)) + cdd * dd[i] * dd[j]);
+        kp[i * 3 + j] += off ? KT(0) : v;
+      }
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename KT>
+__global__ void k_fixed_identity(KT* __restrict__ K, const int32_t* __restrict__ diag,
+                                 const uint8_t* __restrict__ mask, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const uint32_t m = mask[i];
+  if (!m || diag[i] < 0) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if ((m >> d) & 1u) K[9 * (int64_t)diag[i] + 4 * d] = KT(1);
+}
+
+// Inverse of each 3x3 diagonal block (computed in FP64, stored in KT).
+template <typename KT>
+__global__ void k_block_jacobi(const KT* __restrict__ K, const int32_t* __restrict__ diag,
+                               KT* __restrict__ Dinv, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  double A[9];
+  KT* o = Dinv + 9 * i;
+  if (diag[i] < 0) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) o[q] = (q % 4) == 0 ? KT(1) : KT(0);
+    return;
+  }
+  const KT* k = K + 9 * (int64_t)diag[i];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) A[q] = (double)k[q];
+  double c0 = A[4] * A[8] - A[5] * A[7], c1 = A[5] * A[6] - A[3] * A[8], c2 = A[3] * A[7] - A[4] * A[6];
+  double det = A[0] * c0 + A[1] * c1 + A[2] * c2;
+  double r = 1.0 / det;
+  o[0] = (KT)(c0 * r);
+  o[1] = (KT)((A[2] * A[7] - A[1] * A[8]) * r);
+  o[2] = (KT)((A[1] * A[5] - A[2] * A[4]) * r);
+  o[3] = (KT)(c1 * r);
+  o[4] = (KT)((A[0] * A[8] - A[2] * A[6]) * r);
+  o[5] = (KT)((A[2] * A[3] - A[0] * A[5]) * r);
+  o[6] = (KT)(c2 * r);
+  o[7] = (KT)((A[1] * A[6] - A[0] * A[7]) * r);
+  o[8] = (KT)((A[0] * A[4] - A[1] * A[3]) * r);
+}
+
+// Per-element outputs in FP64 (the batch form of internal_forces).
+__global__ void k_tet_outputs(TetArgs ta, const double* __restrict__ U,
+                              const int32_t* __restrict__ perm, double* __restrict__ f,
+                              double* __restrict__ K, double* __restrict__ stress,
+                              double* __restrict__ strain) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ta.ne) return;
+  int32_t nd[4];
+  double n[4][3], u[4][3], V, p1;
+  tet_load(ta, e, U, nd, n, V, u, p1);
+  TetEval ev;
+  tet_eval(ta.mat, ta.p0, p1, n, u, ev);
+  const int64_t o = perm[e];
+  if (f) {
+    for (int a = 0; a < 4; ++a) {
+      double fa[3];
+      tet_force_row(ev, n, V, a, fa);
+      for (int d = 0; d < 3; ++d) f[o * 12 + a * 3 + d] = fa[d];
+    }
+  }
+  if (K) {
+    TetBlocks<double> tb;
+    tb.load(ev, n, V);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        double blk[9];
+        tb.block(a, b, blk);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) K[o * 144 + (a * 3 + i) * 12 + b * 3 + j] = blk[i * 3 + j];
+      }
+  }
+  if (ta.mat == ONSAS_MAT_LINEAR) {
+    // sigma and eps = sym(grad u)
+    if (stress)
+      for (int q = 0; q < 9; ++q) stress[o * 9 + q] = ev.S[q];
+    if (strain) {
+      double H[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          H[i * 3 + j] = u[0][i] * n[0][j] + u[1][i] * n[1][j] + u[2][i] * n[2][j] + u[3][i] * n[3][j];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) strain[o * 9 + i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+    }
+  } else {
+    // P = F S, C = F'F (Tetrahedrons.jl:217-221)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double p = 0, c = 0;
+        for (int k = 0; k < 3; ++k) {
+          p += ev.F[i * 3 + k] * ev.S[k * 3 + j];
+          c += ev.F[k * 3 + i] * ev.F[k * 3 + j];
+        }
+        if (stress) stress[o * 9 + i * 3 + j] = p;
+        if (strain) strain[o * 9 + i * 3 + j] = c;
+      }
+  }
+}
+
+__global__ void k_truss_outputs(TrussArgs ta, const double* __restrict__ U,
+                                const int32_t* __restrict__ perm, double* __restrict__ f,
+                                double* __restrict__ K, double* __restrict__ stress,
+                                double* __restrict__ strain) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ta.ne) return;
+  const int64_t ne = ta.ne;
+  const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
+  double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
+  const double l0 = ta.geo[3 * ne + e], A = ta.geo[4 * ne + e];
+  double du[3];
+  for (int d = 0; d < 3; ++d) du[d] = U[3 * (int64_t)nd1 + d] - U[3 * (int64_t)nd0 + d];
+  TrussEval tv;
+  truss_eval(ta.strain, ta.E, A, d0, l0, du, tv);
+  const int64_t o = perm[e];
+  if (f)
+    for (int a = 0; a < 2; ++a)
+      for (int d = 0; d < 3; ++d) f[o * 6 + a * 3 + d] = (a ? 1.0 : -1.0) * tv.cf * tv.d[d];
+  if (K)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            K[o * 36 + (a * 3 + i) * 6 + b * 3 + j] =
+                (a == b ? 1.0 : -1.0) * ((i == j ? tv.cI : 0.0) + tv.cdd * tv.d[i] * tv.d[j]);
+  if (stress) {
+    for (int q = 0; q < 9; ++q) stress[o * 9 + q] = 0.0;
+    stress[o * 9] = tv.sigma;
+  }
+  if (strain) {
+    for (int q = 0; q < 9; ++q) strain[o * 9 + q] = 0.0;
+    strain[o * 9] = tv.eps;
+  }
+}
+
+// f_ext of node i = nodal + sum over its pressure faces (fixed order) of
+// p(t) (-n) A / 3 (LocalLoadBoundaryConditions.jl:36-56, TriangularFaces.jl:44-65),
+// duplicate dofs summed as in StructuralBoundaryConditions.jl:195-220.
+__global__ void k_fext(const double* __restrict__ x, const double* __restrict__ y,
+                       const double* __restrict__ z, const int32_t* __restrict__ fconn,
+                       const int32_t* __restrict__ fgrp, const int32_t* __restrict__ nf_ptr,
+                       const int32_t* __restrict__ nf_idx, const double* __restrict__ pressures,
+                       const double* __restrict__ nodal, double* __restrict__ fext, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  double acc[3] = {0, 0, 0};
+  for (int32_t k = nf_ptr[i]; k < nf_ptr[i + 1]; ++k) {
+    const int32_t f = nf_idx[k];
+    const int32_t a = fconn[3 * f], b = fconn[3 * f + 1], c = fconn[3 * f + 2];
+    const double e1[3] = {x[b] - x[a], y[b] - y[a], z[b] - z[a]};
+    const double e2[3] = {x[c] - x[a], y[c] - y[a], z[c] - z[a]};
+    const double av[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                          0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
+    const double A = sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+    const double p = pressures[fgrp[f]];
+    for (int d = 0; d < 3; ++d) acc[d] += p * (-(av[d] / A)) * A / 3;
+  }
+  for (int d = 0; d < 3; ++d) fext[3 * i + d] = nodal[3 * i + d] + acc[d];
+}
+
+// dJ/dG_e = -lambda_e . d f_e / d G_e with
+//   NeoHookean: dS/dG = I - C^-1  ->  d f_a / dG = V (F - F^-T) n_a
+//   SVK:        dS/dG = 2E        ->  d f_a / dG = 2 V F E n_a
+template <typename KT>
+__global__ void k_adjoint_shear(TetArgs ta, const double* __restrict__ U,
+                                const KT* __restrict__ lam, const int32_t* __restrict__ perm,
+                                double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ta.ne) return;
+  int32_t nd[4];
+  double n[4][3], u[4][3], V, p1;
+  tet_load(ta, e, U, nd, n, V, u, p1);
+  TetEval ev;
+  tet_eval(ta.mat, ta.p0, p1, n, u, ev);
+  if (ta.mat == ONSAS_MAT_NEOHOOKEAN) {
+    // F^-T n_a is ev.v[a]
+    double s = 0;
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) {
+        double Fn = ev.F[i * 3] * n[a][0] + ev.F[i * 3 + 1] * n[a][1] + ev.F[i * 3 + 2] * n[a][2];
+        s += (double)lam[3 * (int64_t)nd[a] + i] * (Fn - ev.v[a][i]);
+      }
+    out[perm[e]] = -V * s;
+    return;
+  }
+  // SVK: E = (F'F - I)/2
+  double D[9];  // d(F S)/dG = 2 F E
+  double E[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      E[i * 3 + j] = 0.5 * (ev.F[i] * ev.F[j] + ev.F[3 + i] * ev.F[3 + j] + ev.F[6 + i] * ev.F[6 + j] -
+                            (i == j ? 1.0 : 0.0));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += ev.F[i * 3 + k] * E[k * 3 + j];
+      D[i * 3 + j] = 2 * s;
+    }
+  double s = 0;
+  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 3; ++i) {
+      double Dn = D[i * 3] * n[a][0] + D[i * 3 + 1] * n[a][1] + D[i * 3 + 2] * n[a][2];
+      s += (double)lam[3 * (int64_t)nd[a] + i] * Dn;
+    }
+  out[perm[e]] = -V * s;
+}
+
+TetArgs tet_args(const ElemGroup& g, int64_t e0, int64_t e1) {
+  TetArgs ta;
+  ta.conn = g.d_conn;
+  ta.geo = g.d_geo;
+  ta.slot = g.d_slot;
+  ta.G = g.d_G;
+  ta.ne = g.ne;
+  ta.e0 = e0;
+  ta.e1 = e1;
+  ta.mat = g.mat;
+  ta.p0 = g.p0;
+  ta.p1 = g.p1;
+  return ta;
+}
+
+TrussArgs truss_args(const ElemGroup& g, int64_t e0, int64_t e1) {
+  TrussArgs ta;
+  ta.conn = g.d_conn;
+  ta.geo = g.d_geo;
+  ta.slot = g.d_slot;
+  ta.ne = g.ne;
+  ta.e0 = e0;
+  ta.e1 = e1;
+  ta.strain = g.mat;
+  ta.E = g.p0;
+  return ta;
+}
+
+inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+template <typename KT>
+int assemble_t(Ctx* c) {
+  hipStream_t s = c->stream;
+  KT* K = (KT*)c->d_K;
+  const int64_t n9 = 9 * c->nnzb;
+  hipLaunchKernelGGL(k_fill<KT>, dim3(2048), dim3(256), 0, s, K, n9, KT(0));
+  hipLaunchKernelGGL(k_fill<double>, dim3(512), dim3(256), 0, s, c->d_fint, 3 * c->nn, 0.0);
+  hipLaunchKernelGGL(k_fixed_identity<KT>, dim3(grid_for(c->nn, 256)), dim3(256), 0, s, K,
+                     c->d_diag, c->d_fixmask, c->nn);
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  for (const ElemGroup& g : c->groups) {
+    const int ncol = (int)g.color_off.size() - 1;
+    for (int k = 0; k < ncol; ++k) {
+      const int64_t e0 = g.color_off[k], e1 = g.color_off[k + 1];
+      if (e1 <= e0) continue;
+      if (g.type == kTet) {
+        hipLaunchKernelGGL(k_tet_scatter<KT>, dim3(grid_for(4 * (e1 - e0), kScatterThreads)),
+                           dim3(kScatterThreads), 0, s, tet_args(g, e0, e1), c->d_U, c->d_fint, K,
+                           c->d_fixmask);
+      } else {
+        hipLaunchKernelGGL(k_truss_scatter<KT>, dim3(grid_for(2 * (e1 - e0), kScatterThreads)),
+                           dim3(kScatterThreads), 0, s, truss_args(g, e0, e1), c->d_U, c->d_fint,
+                           K, c->d_fixmask);
+      }
+    }
+  }
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[2], s));
+  hipLaunchKernelGGL(k_block_jacobi<KT>, dim3(grid_for(c->nn, 256)), dim3(256), 0, s, K, c->d_diag,
+                     (KT*)c->d_Dinv, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+int launch_assemble(Ctx* c) {
+  return c->precision == ONSAS_PRECISION_FP64 ? assemble_t<double>(c) : assemble_t<float>(c);
+}
+
+int launch_element_outputs(Ctx* c, int gi, const double* U, double* f, double* K, double* stress,
+                           double* strain) {
+  const ElemGroup& g = c->groups[gi];
+  if (g.type == kTet)
+    hipLaunchKernelGGL(k_tet_outputs, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
+                       tet_args(g, 0, g.ne), U, g.d_perm, f, K, stress, strain);
+  else
+    hipLaunchKernelGGL(k_truss_outputs, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
+                       truss_args(g, 0, g.ne), U, g.d_perm, f, K, stress, strain);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_fext(Ctx* c) {
+  hipLaunchKernelGGL(k_fext, dim3(grid_for(c->nn, 256)), dim3(256), 0, c->stream, c->d_x, c->d_y,
+                     c->d_z, c->d_face_conn, c->d_face_grp, c->d_nf_ptr, c->d_nf_idx,
+                     c->d_pressures, c->d_nodal, c->d_fext, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_adjoint_shear(Ctx* c, int gi, const void* lam, double* out) {
+  const ElemGroup& g = c->groups[gi];
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_adjoint_shear<double>, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
+                       tet_args(g, 0, g.ne), c->d_U, (const double*)lam, g.d_perm, out);
+  else
+    hipLaunchKernelGGL(k_adjoint_shear<float>, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
+                       tet_args(g, 0, g.ne), c->d_U, (const float*)lam, g.d_perm, out);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+}  // namespace onsas

@@ -1,0 +1,194 @@
+// engine.hpp -- internal declarations of libonsas_hip (not part of the C ABI).
+//
+// Device data layout (all in HBM, one context per GPU):
+//   nodes     : x, y, z FP64 SoA (reference coordinates; only setup and face loads read them)
+//   U, f_int, f_ext, residual : FP64, dof-interleaved (dof 3i+d of node i)
+//   tangent   : 3x3 block CSR over nodes; row_ptr/col int32; values KT[9] per block,
+//               row-major inside the block (KT = float in mixed mode, double in FP64 mode)
+//   elements  : per group, SoA arrays in colour-sorted order:
+//               conn[k][ne] int32, geo[q][ne] FP64 (tet: dN 12 + V; truss: d0 3 + l0 + A),
+//               slot[a*nnode+b][ne] int32 (block index of (a, b) in the tangent)
+//   PCG       : x, r, z, p0, p1, q in KT, per-node block-Jacobi inverse KT[9]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/onsas_hip.h"
+
+namespace onsas {
+
+constexpr int kNodesTet = 4;
+constexpr int kNodesTruss = 2;
+constexpr int kRedThreads = 1024;  // threads per workgroup of every reduction-producing kernel
+constexpr int kRowGroup = 16;      // lanes per block row in the SpMV kernels
+constexpr int kMaxPartials = 1024; // upper bound of the reduction grid (multiple of 8)
+
+enum GroupType { kTet = 0, kTruss = 1 };
+
+struct ElemGroup {
+  int type = kTet;
+  int mat = 0;          // ONSAS_MAT_* for tets, strain model for trusses
+  double p0 = 0, p1 = 0;  // tet: material parameters; truss: E
+  int64_t ne = 0;
+  int nnode = 4;
+  bool per_elem_G = false;
+  std::vector<int32_t> conn;   // user order, element-major
+  std::vector<double> area;    // truss areas (user order)
+  std::vector<double> Gel;     // per-element G (user order)
+  // colouring (filled by finalize)
+  std::vector<int64_t> color_off;  // ncolors + 1 offsets into the sorted order
+  std::vector<int32_t> perm;       // sorted position -> user element id
+  // device arrays (sorted order, SoA with stride ne)
+  int32_t* d_conn = nullptr;
+  double* d_geo = nullptr;
+  int32_t* d_slot = nullptr;
+  double* d_G = nullptr;
+  int32_t* d_perm = nullptr;
+};
+
+struct PcgState {     // device-resident scalars of the PCG iteration
+  double tol2;        // (rtol ||b||)^2
+  double rr_last;     // ||r_k||^2 at the last convergence check
+  double bb;          // ||b||^2
+  int32_t k;          // iterations completed
+  int32_t done;       // 1 once converged or k reached maxiter
+  int32_t iters;      // iteration count when done
+  int32_t breakdown;  // 1 if p'Ap <= 0 or non-finite
+  int32_t breakdown_k;
+  int32_t maxiter;
+  int32_t converged;  // 1 if ||r_k|| <= rtol ||b||
+  int32_t pad;
+};
+
+struct NewtonStats {  // device-resident FP64 norms of one Newton step
+  double r2;          // ||f_ext[free] - f_int[free]||^2
+  double fext2;       // ||f_ext||^2
+  double dU2;         // ||dU||^2
+  double U2;          // ||U||^2 before the update
+};
+
+struct Scalars {      // one allocation so a single copy returns everything
+  PcgState pcg;
+  NewtonStats nwt;
+};
+
+struct Ctx {
+  int precision = ONSAS_PRECISION_MIXED;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int injected = 0;
+  bool finalized = false;
+  size_t kt = 4;  // bytes per tangent / PCG scalar
+
+  // model (host)
+  int64_t nn = 0;
+  std::vector<double> X, Y, Z;
+  std::vector<ElemGroup> groups;
+  std::vector<std::vector<int32_t>> pressure_faces;  // per pressure group, 3*nf
+  std::vector<int64_t> fixed;
+  std::vector<uint8_t> fixmask;  // per node, bit d = dof 3i+d fixed
+  int64_t nfree = 0;
+
+  // pattern (host copy kept for export)
+  std::vector<int32_t> row_ptr, col, diag;
+  int64_t nnzb = 0;
+
+  // device
+  double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+  uint8_t* d_fixmask = nullptr;
+  int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_diag = nullptr;
+  void* d_K = nullptr;     // nnzb*9 KT
+  void* d_Dinv = nullptr;  // nn*9 KT
+  double *d_U = nullptr, *d_Uasm = nullptr, *d_fint = nullptr, *d_fext = nullptr, *d_res = nullptr;
+  double* d_nodal = nullptr;
+  void *d_px = nullptr, *d_pr = nullptr, *d_pz = nullptr, *d_pp0 = nullptr, *d_pp1 = nullptr,
+       *d_pq = nullptr;  // PCG vectors (KT)
+  double* d_part = nullptr;  // partial-sum arrays, kPartArrays x kMaxPartials
+  Scalars* d_sc = nullptr;
+  Scalars* h_sc = nullptr;   // pinned
+  // pressure faces
+  int32_t *d_face_conn = nullptr, *d_face_grp = nullptr;  // all faces, 3*nf and nf
+  int32_t *d_nf_ptr = nullptr, *d_nf_idx = nullptr;       // node -> faces CSR
+  double* d_pressures = nullptr;
+  int64_t nfaces = 0;
+
+  // reduction grid
+  int np = 8;          // workgroups of reduction kernels (multiple of 8)
+  int64_t chunk = 1;   // rows (nodes) per logical workgroup
+
+  // linear solver settings
+  onsas_linear_settings lin{};
+
+  // PCG graph
+  hipGraphExec_t pcg_graph = nullptr;
+  int pcg_graph_iters = 0;
+
+  // GMRES workspace (allocated on first use)
+  void* d_gm_V = nullptr;   // (m+1) * n KT
+  void* d_gm_w = nullptr;   // n KT
+  double* d_gm_h = nullptr; // partial dots
+  int gm_m = 0;
+
+  // timers
+  hipEvent_t ev[8] = {};
+  hipEvent_t ev_spmv[2 * 32] = {};  // brackets of the sampled SpMV launches
+  onsas_timings tm{};
+  double last_assemble_ms = 0;
+  bool assembled = false;
+};
+
+enum PartArray {  // rows of d_part
+  kPartRZ0 = 0, kPartRZ1, kPartRR, kPartPQ, kPartBB, kPartR2, kPartF2, kPartDU2, kPartU2,
+  kPartArrays
+};
+
+inline double* part(Ctx* c, int a) { return c->d_part + (size_t)a * kMaxPartials; }
+
+// ---- setup.cpp ----
+int finalize_model(Ctx* c);
+
+// ---- element_kernels.hip ----
+int launch_assemble(Ctx* c);                                    // zero + scatter + Dinv
+int launch_element_outputs(Ctx* c, int g, const double* U, double* f, double* K, double* stress,
+                           double* strain);
+int launch_fext(Ctx* c);                                        // nodal + pressure faces
+int launch_adjoint_shear(Ctx* c, int g, const void* lambda_kt, double* dJdG_dev);
+
+// ---- solver_kernels.hip ----
+int launch_residual(Ctx* c);                                    // d_res, partial R2/F2
+int launch_pcg_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
+                     int* breakdown);
+int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged);
+int launch_update(Ctx* c);                                      // U += x, partial DU2/U2
+int launch_finalize_newton(Ctx* c);                             // partials -> d_sc->nwt
+int launch_matvec(Ctx* c, const double* x_dev, double* y_dev);
+int launch_x_to_double(Ctx* c, double* out_dev);               // PCG x (KT) -> FP64 (fixed = 0)
+int launch_convert_K(Ctx* c, double* out_dev);
+
+// ---- error helpers ----
+int set_err(Ctx* c, int code, const std::string& msg);
+
+}  // namespace onsas
+
+#define ONSAS_HIP_TRY(c, expr)                                                            \
+  do {                                                                                    \
+    hipError_t e__ = (expr);                                                              \
+    if (e__ != hipSuccess)                                                                \
+      return ::onsas::set_err((c), ONSAS_ERR_HIP,                                         \
+                              std::string(#expr " failed: ") + hipGetErrorString(e__));  \
+  } while (0)
+
+#define ONSAS_TRY(expr)        \
+  do {                         \
+    int s__ = (expr);          \
+    if (s__ != ONSAS_OK) return s__; \
+  } while (0)
+
+struct onsas_ctx {
+  onsas::Ctx impl;
+};

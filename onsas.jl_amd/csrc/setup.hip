@@ -1,0 +1,267 @@
+// setup.hip -- one-time host work of onsas_finalize: reference-configuration
+// shape gradients, element colouring, the 3x3 block-CSR pattern and the
+// element->block slot maps, then the upload of everything to HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "engine.hpp"
+
+namespace onsas {
+
+namespace {
+
+template <typename T>
+int dev_upload(Ctx* c, T** dst, const T* src, size_t count) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  ONSAS_HIP_TRY(c, hipMalloc((void**)dst, bytes));
+  if (count) ONSAS_HIP_TRY(c, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return ONSAS_OK;
+}
+
+int dev_alloc(Ctx* c, void** dst, size_t bytes) {
+  ONSAS_HIP_TRY(c, hipMalloc(dst, std::max<size_t>(bytes, 16)));
+  ONSAS_HIP_TRY(c, hipMemset(*dst, 0, std::max<size_t>(bytes, 16)));
+  return ONSAS_OK;
+}
+
+// Shape-function gradients of a tetrahedron in the reference configuration:
+// J = X dXdz^T, V = det J / 6, dN = inv(J)^T dXdz (src/Entities/Tetrahedrons.jl:
+// 129-138, 193-197, 269-271).  dN is node-major (n*3 + j).
+bool tet_gradients(const double X[12], double dN[12], double* V) {
+  static const double D[3][4] = {{1, -1, 0, 0}, {0, -1, 0, 1}, {0, -1, 1, 0}};
+  double J[9];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) {
+      double s = 0;
+      for (int n = 0; n < 4; ++n) s += X[n * 3 + i] * D[k][n];
+      J[i * 3 + k] = s;
+    }
+  double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+               J[2] * (J[3] * J[7] - J[4] * J[6]);
+  *V = det / 6.0;
+  if (!(*V > 0)) return false;
+  double Ji[9] = {(J[4] * J[8] - J[5] * J[7]) / det, (J[2] * J[7] - J[1] * J[8]) / det,
+                  (J[1] * J[5] - J[2] * J[4]) / det, (J[5] * J[6] - J[3] * J[8]) / det,
+                  (J[0] * J[8] - J[2] * J[6]) / det, (J[2] * J[3] - J[0] * J[5]) / det,
+                  (J[3] * J[7] - J[4] * J[6]) / det, (J[1] * J[6] - J[0] * J[7]) / det,
+                  (J[0] * J[4] - J[1] * J[3]) / det};
+  for (int j = 0; j < 3; ++j)
+    for (int n = 0; n < 4; ++n) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += Ji[k * 3 + j] * D[k][n];
+      dN[n * 3 + j] = s;
+    }
+  return true;
+}
+
+// Greedy first-fit colouring of the element conflict graph (two elements
+// conflict when they share a node).  Elements keep their user order inside a
+// colour, so structured meshes stay spatially coherent within each launch.
+int color_group(Ctx* c, ElemGroup& g) {
+  const int nw = 4;  // 256 colours max
+  std::vector<uint64_t> used((size_t)c->nn * nw, 0);
+  std::vector<int16_t> color(g.ne);
+  int ncol = 0;
+  for (int64_t e = 0; e < g.ne; ++e) {
+    uint64_t m[nw] = {0, 0, 0, 0};
+    for (int a = 0; a < g.nnode; ++a) {
+      const uint64_t* u = &used[(size_t)g.conn[e * g.nnode + a] * nw];
+      for (int w = 0; w < nw; ++w) m[w] |= u[w];
+    }
+    int col = -1;
+    for (int w = 0; w < nw && col < 0; ++w)
+      if (~m[w]) col = w * 64 + __builtin_ctzll(~m[w]);
+    if (col < 0) return set_err(c, ONSAS_ERR_UNSUPPORTED, "element colouring needs > 256 colours");
+    color[e] = (int16_t)col;
+    ncol = std::max(ncol, col + 1);
+    for (int a = 0; a < g.nnode; ++a)
+      used[(size_t)g.conn[e * g.nnode + a] * nw + col / 64] |= 1ull << (col % 64);
+  }
+  g.color_off.assign(ncol + 1, 0);
+  for (int64_t e = 0; e < g.ne; ++e) g.color_off[color[e] + 1]++;
+  for (int k = 0; k < ncol; ++k) g.color_off[k + 1] += g.color_off[k];
+  std::vector<int64_t> pos(g.color_off.begin(), g.color_off.end() - 1);
+  g.perm.resize(g.ne);
+  for (int64_t e = 0; e < g.ne; ++e) g.perm[pos[color[e]]++] = (int32_t)e;
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+int finalize_model(Ctx* c) {
+  if (c->nn <= 0) return set_err(c, ONSAS_ERR_STATE, "onsas_set_nodes was not called");
+  if (c->groups.empty()) return set_err(c, ONSAS_ERR_STATE, "no element group was added");
+  const int64_t nn = c->nn, n = 3 * nn;
+  if (nn >= (int64_t(1) << 31) / 3) return set_err(c, ONSAS_ERR_UNSUPPORTED, "too many nodes");
+
+  // fixed-dof mask
+  c->fixmask.assign(nn, 0);
+  for (int64_t d : c->fixed) c->fixmask[d / 3] |= (uint8_t)(1u << (d % 3));
+  int64_t nfixed = 0;
+  for (uint8_t m : c->fixmask) nfixed += __builtin_popcount(m);
+  c->nfree = n - nfixed;
+
+  // colouring
+  for (auto& g : c->groups) ONSAS_TRY(color_group(c, g));
+
+  // block pattern: every (node_a, node_b) pair of every element
+  std::vector<int64_t> cnt(nn + 1, 0);
+  for (auto& g : c->groups)
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int a = 0; a < g.nnode; ++a) cnt[g.conn[e * g.nnode + a] + 1] += g.nnode;
+  for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int32_t> cand(cnt[nn]);
+  {
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (auto& g : c->groups)
+      for (int64_t e = 0; e < g.ne; ++e)
+        for (int a = 0; a < g.nnode; ++a) {
+          int32_t r = g.conn[e * g.nnode + a];
+          for (int b = 0; b < g.nnode; ++b) cand[pos[r]++] = g.conn[e * g.nnode + b];
+        }
+  }
+  c->row_ptr.assign(nn + 1, 0);
+  for (int64_t i = 0; i < nn; ++i) {
+    auto b = cand.begin() + cnt[i], e = cand.begin() + cnt[i + 1];
+    std::sort(b, e);
+    auto u = std::unique(b, e);
+    c->row_ptr[i + 1] = c->row_ptr[i] + (int32_t)(u - b);
+  }
+  c->nnzb = c->row_ptr[nn];
+  if (c->nnzb >= (int64_t(1) << 31) / 9)
+    return set_err(c, ONSAS_ERR_UNSUPPORTED, "tangent too large for 32-bit slot indices");
+  c->col.resize(c->nnzb);
+  c->diag.assign(nn, -1);
+  for (int64_t i = 0; i < nn; ++i) {
+    std::copy(cand.begin() + cnt[i], cand.begin() + cnt[i] + (c->row_ptr[i + 1] - c->row_ptr[i]),
+              c->col.begin() + c->row_ptr[i]);
+  }
+  for (int64_t i = 0; i < nn; ++i) {
+    auto b = c->col.begin() + c->row_ptr[i], e = c->col.begin() + c->row_ptr[i + 1];
+    auto it = std::lower_bound(b, e, (int32_t)i);
+    if (it != e && *it == i) c->diag[i] = (int32_t)(it - c->col.begin());
+  }
+  // nodes that no element touches get no diagonal block: they must be fixed
+  for (int64_t i = 0; i < nn; ++i)
+    if (c->diag[i] < 0 && c->fixmask[i] != 7)
+      return set_err(c, ONSAS_ERR_ARG,
+                     "node " + std::to_string(i) + " belongs to no element and is not fixed");
+  auto slot_of = [&](int32_t r, int32_t cc) -> int32_t {
+    auto b = c->col.begin() + c->row_ptr[r], e = c->col.begin() + c->row_ptr[r + 1];
+    return (int32_t)(std::lower_bound(b, e, cc) - c->col.begin());
+  };
+
+  // per-group device arrays in colour-sorted order
+  for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+    ElemGroup& g = c->groups[gi];
+    const int64_t ne = g.ne;
+    const int nn_e = g.nnode;
+    std::vector<int32_t> conn((size_t)nn_e * ne), slot((size_t)nn_e * nn_e * ne);
+    int ngeo = (g.type == kTet) ? 13 : 5;
+    std::vector<double> geo((size_t)ngeo * ne);
+    std::vector<double> Gs;
+    if (g.per_elem_G) Gs.resize(ne);
+    for (int64_t s = 0; s < ne; ++s) {
+      int64_t e = g.perm[s];
+      const int32_t* ce = &g.conn[e * nn_e];
+      for (int a = 0; a < nn_e; ++a) conn[(size_t)a * ne + s] = ce[a];
+      for (int a = 0; a < nn_e; ++a)
+        for (int b = 0; b < nn_e; ++b) slot[(size_t)(a * nn_e + b) * ne + s] = slot_of(ce[a], ce[b]);
+      if (g.type == kTet) {
+        double X[12], dN[12], V;
+        for (int a = 0; a < 4; ++a) {
+          X[a * 3 + 0] = c->X[ce[a]];
+          X[a * 3 + 1] = c->Y[ce[a]];
+          X[a * 3 + 2] = c->Z[ce[a]];
+        }
+        if (!tet_gradients(X, dN, &V))
+          return set_err(c, ONSAS_ERR_NEGATIVE_VOLUME,
+                         "Element with negative volume, check connectivity (group " +
+                             std::to_string(gi) + ", element " + std::to_string(e) + ")");
+        for (int q = 0; q < 12; ++q) geo[(size_t)q * ne + s] = dN[q];
+        geo[(size_t)12 * ne + s] = V;
+      } else {
+        double d0[3] = {c->X[ce[1]] - c->X[ce[0]], c->Y[ce[1]] - c->Y[ce[0]],
+                        c->Z[ce[1]] - c->Z[ce[0]]};
+        double l0 = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+        if (!(l0 > 0))
+          return set_err(c, ONSAS_ERR_ARG, "truss element " + std::to_string(e) + " has zero length");
+        for (int q = 0; q < 3; ++q) geo[(size_t)q * ne + s] = d0[q];
+        geo[(size_t)3 * ne + s] = l0;
+        geo[(size_t)4 * ne + s] = g.area[e];
+      }
+      if (g.per_elem_G) Gs[s] = g.Gel[e];
+    }
+    ONSAS_TRY(dev_upload(c, &g.d_conn, conn.data(), conn.size()));
+    ONSAS_TRY(dev_upload(c, &g.d_slot, slot.data(), slot.size()));
+    ONSAS_TRY(dev_upload(c, &g.d_geo, geo.data(), geo.size()));
+    ONSAS_TRY(dev_upload(c, &g.d_perm, g.perm.data(), g.perm.size()));
+    if (g.per_elem_G) ONSAS_TRY(dev_upload(c, &g.d_G, Gs.data(), Gs.size()));
+  }
+
+  // nodes, pattern, masks
+  ONSAS_TRY(dev_upload(c, &c->d_x, c->X.data(), nn));
+  ONSAS_TRY(dev_upload(c, &c->d_y, c->Y.data(), nn));
+  ONSAS_TRY(dev_upload(c, &c->d_z, c->Z.data(), nn));
+  ONSAS_TRY(dev_upload(c, &c->d_fixmask, c->fixmask.data(), nn));
+  ONSAS_TRY(dev_upload(c, &c->d_row_ptr, c->row_ptr.data(), nn + 1));
+  ONSAS_TRY(dev_upload(c, &c->d_col, c->col.data(), c->nnzb));
+  ONSAS_TRY(dev_upload(c, &c->d_diag, c->diag.data(), nn));
+
+  // pressure faces: node -> incident faces CSR (pressure-group order, then face order)
+  {
+    std::vector<int32_t> fconn, fgrp;
+    for (size_t pg = 0; pg < c->pressure_faces.size(); ++pg) {
+      const auto& f = c->pressure_faces[pg];
+      fconn.insert(fconn.end(), f.begin(), f.end());
+      fgrp.insert(fgrp.end(), f.size() / 3, (int32_t)pg);
+    }
+    c->nfaces = (int64_t)fgrp.size();
+    std::vector<int32_t> ptr(nn + 1, 0);
+    for (int64_t f = 0; f < c->nfaces; ++f)
+      for (int a = 0; a < 3; ++a) ptr[fconn[f * 3 + a] + 1]++;
+    for (int64_t i = 0; i < nn; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> idx(ptr[nn]);
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t f = 0; f < c->nfaces; ++f)
+      for (int a = 0; a < 3; ++a) idx[pos[fconn[f * 3 + a]]++] = (int32_t)f;
+    ONSAS_TRY(dev_upload(c, &c->d_face_conn, fconn.data(), fconn.size()));
+    ONSAS_TRY(dev_upload(c, &c->d_face_grp, fgrp.data(), fgrp.size()));
+    ONSAS_TRY(dev_upload(c, &c->d_nf_ptr, ptr.data(), ptr.size()));
+    ONSAS_TRY(dev_upload(c, &c->d_nf_idx, idx.data(), idx.size()));
+    ONSAS_TRY(dev_alloc(c, (void**)&c->d_pressures,
+                        sizeof(double) * std::max<size_t>(c->pressure_faces.size(), 1)));
+  }
+
+  // state and solver vectors
+  c->kt = (c->precision == ONSAS_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+  ONSAS_TRY(dev_alloc(c, &c->d_K, c->kt * 9 * (size_t)c->nnzb));
+  ONSAS_TRY(dev_alloc(c, &c->d_Dinv, c->kt * 9 * (size_t)nn));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_U, sizeof(double) * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_Uasm, sizeof(double) * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_fint, sizeof(double) * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_fext, sizeof(double) * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_res, sizeof(double) * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_nodal, sizeof(double) * n));
+  for (void** v : {&c->d_px, &c->d_pr, &c->d_pz, &c->d_pp0, &c->d_pp1, &c->d_pq})
+    ONSAS_TRY(dev_alloc(c, v, c->kt * n));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_part, sizeof(double) * kPartArrays * kMaxPartials));
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_sc, sizeof(Scalars)));
+  ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
+
+  // reduction grid: >= 64 rows per workgroup, at most kMaxPartials/2 workgroups, multiple of 8
+  int64_t want = (nn + 63) / 64;
+  int64_t np = std::min<int64_t>(512, std::max<int64_t>(8, ((want + 7) / 8) * 8));
+  c->np = (int)np;
+  c->chunk = (nn + np - 1) / np;
+
+  for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
+  for (auto& e : c->ev_spmv) ONSAS_HIP_TRY(c, hipEventCreate(&e));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->finalized = true;
+  return ONSAS_OK;
+}
+
+}  // namespace onsas

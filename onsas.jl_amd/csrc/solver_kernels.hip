@@ -1,0 +1,540 @@
+// solver_kernels.hip -- FP64 Newton residual / norms / update and the
+// block-Jacobi preconditioned CG on the 3x3 block-CSR tangent.  gfx950, wave64.
+//
+// Reductions are deterministic and atomic-free: every reduction-producing
+// kernel runs a fixed grid of np workgroups x 1024 threads; each workgroup
+// reduces with a fixed shuffle tree + fixed-order LDS sum and writes one
+// partial; the consumer kernel (every workgroup of the next launch) re-sums the
+// np partials in index order.  Kernel boundaries give the visibility, so no
+// cross-workgroup fences or counters are needed and the same inputs give the
+// same bits on every run.
+//
+// PCG iteration k is two kernels (pcg_A, pcg_B) with the search direction
+// ping-ponged between two buffers by parity:
+//   pcg_A: beta = rz_k / rz_{k-1};  p_k = z + beta p_{k-1} (formed on the fly for
+//          every gathered column);  q = K p_k;  partial p_k.q;  convergence test
+//          ||r_k||^2 <= rtol^2 ||b||^2 and k >= maxiter.
+//   pcg_B: alpha = rz_k / p_k.q;  x += alpha p_k;  r -= alpha q;  z = D^-1 r;
+//          partials r.z and r.r;  breakdown test p.q <= 0 (indefinite tangent).
+// Chunks of iterations are captured once in a HIP graph and replayed; the host
+// reads the device state between chunks.
+#include <cmath>
+
+#include "engine.hpp"
+
+namespace onsas {
+
+namespace {
+
+constexpr int NT = kRedThreads;
+
+__device__ inline double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += sh[w];
+  return s;
+}
+
+__device__ inline double sum_parts(const double* __restrict__ p, int np, double* sh) {
+  double v = 0;
+  for (int i = threadIdx.x; i < np; i += NT) v += p[i];
+  return block_sum(v, sh);
+}
+
+// XCD-aware row range: consecutive hardware workgroup ids land on different
+// XCDs (round-robin), so logical chunk = (id % 8) * (np / 8) + id / 8 keeps
+// each XCD on a contiguous band of rows (its L2 sees the neighbouring columns).
+__device__ inline void wg_rows(int np, int64_t chunk, int64_t nn, int64_t& r0, int64_t& r1) {
+  const int b = blockIdx.x;
+  const int64_t L = (int64_t)(b & 7) * (np >> 3) + (b >> 3);
+  r0 = L * chunk;
+  r1 = r0 + chunk < nn ? r0 + chunk : nn;
+}
+
+// ---- Newton residual: r = f_ext - f_int on free dofs (FP64) -------------------
+__global__ __launch_bounds__(NT) void k_residual(const double* __restrict__ fext,
+                                                   const double* __restrict__ fint,
+                                                   const uint8_t* __restrict__ mask,
+                                                   double* __restrict__ res, double* __restrict__ pr2,
+                                                   double* __restrict__ pf2, int np, int64_t chunk,
+                                                   int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double r2 = 0, f2 = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    const uint32_t m = mask[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double fe = fext[3 * i + d];
+      const double r = ((m >> d) & 1u) ? 0.0 : fe - fint[3 * i + d];
+      res[3 * i + d] = r;
+      r2 += r * r;
+      f2 += fe * fe;
+    }
+  }
+  r2 = block_sum(r2, sh);
+  f2 = block_sum(f2, sh);
+  if (threadIdx.x == 0) {
+    pr2[blockIdx.x] = r2;
+    pf2[blockIdx.x] = f2;
+  }
+}
+
+// ---- Newton update: U[free] += dU (FP64), norms of dU and of U before ---------
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_update(const KT* __restrict__ x,
+                                                 const uint8_t* __restrict__ mask,
+                                                 double* __restrict__ U, double* __restrict__ pdu,
+                                                 double* __restrict__ pu, int np, int64_t chunk,
+                                                 int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double du2 = 0, u2 = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    const uint32_t m = mask[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double du = ((m >> d) & 1u) ? 0.0 : (double)x[3 * i + d];
+      const double u = U[3 * i + d];
+      du2 += du * du;
+      u2 += u * u;
+      U[3 * i + d] = u + du;
+    }
+  }
+  du2 = block_sum(du2, sh);
+  u2 = block_sum(u2, sh);
+  if (threadIdx.x == 0) {
+    pdu[blockIdx.x] = du2;
+    pu[blockIdx.x] = u2;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_finalize_newton(const double* __restrict__ parts,
+                                                          Scalars* __restrict__ sc, int np) {
+  __shared__ double sh[NT / 64];
+  const double r2 = sum_parts(parts + kPartR2 * kMaxPartials, np, sh);
+  const double f2 = sum_parts(parts + kPartF2 * kMaxPartials, np, sh);
+  const double du2 = sum_parts(parts + kPartDU2 * kMaxPartials, np, sh);
+  const double u2 = sum_parts(parts + kPartU2 * kMaxPartials, np, sh);
+  if (threadIdx.x == 0) {
+    sc->nwt.r2 = r2;
+    sc->nwt.fext2 = f2;
+    sc->nwt.dU2 = du2;
+    sc->nwt.U2 = u2;
+  }
+}
+
+// ---- PCG ------------------------------------------------------------------------
+template <typename KT>
+__device__ inline void jacobi_apply(const KT* __restrict__ D, const KT r[3], KT z[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) z[i] = D[3 * i] * r[0] + D[3 * i + 1] * r[1] + D[3 * i + 2] * r[2];
+}
+
+// r = b (free dofs, KT), x = 0, z = D^-1 r; partials r.z, r.r
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_pcg_init(const double* __restrict__ b,
+                                                   const uint8_t* __restrict__ mask,
+                                                   const KT* __restrict__ Dinv, KT* __restrict__ x,
+                                                   KT* __restrict__ r, KT* __restrict__ z,
+                                                   double* __restrict__ prz, double* __restrict__ prr,
+                                                   double* __restrict__ pbb, int np, int64_t chunk,
+                                                   int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double rz = 0, rr = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    const uint32_t m = mask[i];
+    KT ri[3], zi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ri[d] = ((m >> d) & 1u) ? KT(0) : (KT)b[3 * i + d];
+    jacobi_apply(Dinv + 9 * i, ri, zi);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[3 * i + d] = KT(0);
+      r[3 * i + d] = ri[d];
+      z[3 * i + d] = zi[d];
+      rz += (double)ri[d] * (double)zi[d];
+      rr += (double)ri[d] * (double)ri[d];
+    }
+  }
+  rz = block_sum(rz, sh);
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    prz[blockIdx.x] = rz;
+    prr[blockIdx.x] = rr;
+    pbb[blockIdx.x] = rr;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_init_finalize(const double* __restrict__ pbb,
+                                                            PcgState* __restrict__ st, double rtol,
+                                                            int maxiter, int np) {
+  __shared__ double sh[NT / 64];
+  const double bb = sum_parts(pbb, np, sh);
+  if (threadIdx.x == 0) {
+    st->tol2 = rtol * rtol * bb;
+    st->bb = bb;
+    st->rr_last = bb;
+    st->k = 0;
+    st->done = 0;
+    st->iters = 0;
+    st->breakdown = 0;
+    st->breakdown_k = -1;
+    st->maxiter = maxiter;
+    st->converged = 0;
+  }
+}
+
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_pcg_A(const int32_t* __restrict__ row_ptr,
+                                                const int32_t* __restrict__ col,
+                                                const KT* __restrict__ val, const KT* __restrict__ z,
+                                                const KT* __restrict__ pold, KT* __restrict__ pnew,
+                                                KT* __restrict__ q, const double* __restrict__ prz_cur,
+                                                const double* __restrict__ prz_prev,
+                                                const double* __restrict__ prr,
+                                                double* __restrict__ ppq, PcgState* __restrict__ st,
+                                                int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[NT / 64];
+  if (st->done | st->breakdown) return;
+  const int k = st->k;
+  const double rz = sum_parts(prz_cur, np, sh);
+  const double rr = sum_parts(prr, np, sh);
+  double beta = 0.0;
+  if (k > 0) beta = rz / sum_parts(prz_prev, np, sh);
+  const bool conv = rr <= st->tol2;
+  if (conv || k >= st->maxiter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->iters = k;
+      st->converged = conv ? 1 : 0;
+      st->rr_last = rr;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_last = rr;
+  const KT bK = (KT)beta;
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  const int g = threadIdx.x / kRowGroup, l = threadIdx.x % kRowGroup;
+  double pq = 0;
+  for (int64_t i = r0 + g; i < r1; i += NT / kRowGroup) {
+    const int32_t rs = row_ptr[i], re = row_ptr[i + 1];
+    KT a0 = 0, a1 = 0, a2 = 0;
+    for (int32_t kk = rs + l; kk < re; kk += kRowGroup) {
+      const int64_t j = col[kk];
+      const KT* v = val + 9 * (int64_t)kk;
+      KT p0 = z[3 * j], p1 = z[3 * j + 1], p2 = z[3 * j + 2];
+      if (k > 0) {
+        p0 += bK * pold[3 * j];
+        p1 += bK * pold[3 * j + 1];
+        p2 += bK * pold[3 * j + 2];
+      }
+      a0 += v[0] * p0 + v[1] * p1 + v[2] * p2;
+      a1 += v[3] * p0 + v[4] * p1 + v[5] * p2;
+      a2 += v[6] * p0 + v[7] * p1 + v[8] * p2;
+    }
+#pragma unroll
+    for (int o = kRowGroup / 2; o > 0; o >>= 1) {
+      a0 += __shfl_xor(a0, o, kRowGroup);
+      a1 += __shfl_xor(a1, o, kRowGroup);
+      a2 += __shfl_xor(a2, o, kRowGroup);
+    }
+    if (l == 0) {
+      KT p0 = z[3 * i], p1 = z[3 * i + 1], p2 = z[3 * i + 2];
+      if (k > 0) {
+        p0 += bK * pold[3 * i];
+        p1 += bK * pold[3 * i + 1];
+        p2 += bK * pold[3 * i + 2];
+      }
+      pnew[3 * i] = p0;
+      pnew[3 * i + 1] = p1;
+      pnew[3 * i + 2] = p2;
+      q[3 * i] = a0;
+      q[3 * i + 1] = a1;
+      q[3 * i + 2] = a2;
+      pq += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
+    }
+  }
+  pq = block_sum(pq, sh);
+  if (threadIdx.x == 0) ppq[blockIdx.x] = pq;
+}
+
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_pcg_B(const KT* __restrict__ p, const KT* __restrict__ q,
+                                                const KT* __restrict__ Dinv, KT* __restrict__ x,
+                                                KT* __restrict__ r, KT* __restrict__ z,
+                                                const double* __restrict__ prz_cur,
+                                                const double* __restrict__ ppq,
+                                                double* __restrict__ prz_next,
+                                                double* __restrict__ prr, PcgState* __restrict__ st,
+                                                int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[NT / 64];
+  if (st->done | st->breakdown) return;
+  const double rz = sum_parts(prz_cur, np, sh);
+  const double pq = sum_parts(ppq, np, sh);
+  if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->breakdown = 1;
+      st->breakdown_k = st->k;
+    }
+    return;
+  }
+  const KT alpha = (KT)(rz / pq);
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double rzn = 0, rrn = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    KT ri[3], zi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[3 * i + d] += alpha * p[3 * i + d];
+      ri[d] = r[3 * i + d] - alpha * q[3 * i + d];
+      r[3 * i + d] = ri[d];
+    }
+    jacobi_apply(Dinv + 9 * i, ri, zi);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      z[3 * i + d] = zi[d];
+      rzn += (double)ri[d] * (double)zi[d];
+      rrn += (double)ri[d] * (double)ri[d];
+    }
+  }
+  rzn = block_sum(rzn, sh);
+  rrn = block_sum(rrn, sh);
+  if (threadIdx.x == 0) {
+    prz_next[blockIdx.x] = rzn;
+    prr[blockIdx.x] = rrn;
+    if (blockIdx.x == 0) st->k = st->k + 1;
+  }
+}
+
+// y = K x (FP64 accumulation), for onsas_tangent_matvec
+template <typename KT>
+__global__ __launch_bounds__(256) void k_spmv_f64(const int32_t* __restrict__ row_ptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const KT* __restrict__ val,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ y, int64_t nn) {
+  const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kRowGroup;
+  const int l = threadIdx.x % kRowGroup;
+  double a0 = 0, a1 = 0, a2 = 0;
+  if (i < nn) {
+    for (int32_t kk = row_ptr[i] + l; kk < row_ptr[i + 1]; kk += kRowGroup) {
+      const int64_t j = col[kk];
+      const KT* v = val + 9 * (int64_t)kk;
+      const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+      a0 += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
+      a1 += (double)v[3] * x0 + (double)v[4] * x1 + (double)v[5] * x2;
+      a2 += (double)v[6] * x0 + (double)v[7] * x1 + (double)v[8] * x2;
+    }
+  }
+#pragma unroll
+  for (int o = kRowGroup / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, kRowGroup);
+    a1 += __shfl_xor(a1, o, kRowGroup);
+    a2 += __shfl_xor(a2, o, kRowGroup);
+  }
+  if (i < nn && l == 0) {
+    y[3 * i] = a0;
+    y[3 * i + 1] = a1;
+    y[3 * i + 2] = a2;
+  }
+}
+
+template <typename KT>
+__global__ void k_to_double(const KT* __restrict__ x, const uint8_t* __restrict__ mask,
+                            double* __restrict__ out, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = ((mask[t / 3] >> (t % 3)) & 1u) ? 0.0 : (double)x[t];
+}
+
+template <typename KT>
+__global__ void k_convert(const KT* __restrict__ x, double* __restrict__ out, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (double)x[t];
+}
+
+inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
+// optional events around pcg_A.
+template <typename KT>
+void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
+  hipStream_t s = c->stream;
+  KT* P[2] = {(KT*)c->d_pp0, (KT*)c->d_pp1};
+  double* RZ[2] = {part(c, kPartRZ0), part(c, kPartRZ1)};
+  PcgState* st = &c->d_sc->pcg;
+  if (ea) (void)hipEventRecord(ea, s);
+  hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(NT), 0, s, c->d_row_ptr, c->d_col,
+                     (const KT*)c->d_K, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
+                     RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, c->chunk,
+                     c->nn);
+  if (eb) (void)hipEventRecord(eb, s);
+  hipLaunchKernelGGL(k_pcg_B<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
+                     (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz, RZ[par],
+                     part(c, kPartPQ), RZ[1 - par], part(c, kPartRR), st, c->np, c->chunk, c->nn);
+}
+
+template <typename KT>
+int build_pcg_graph(Ctx* c, int iters) {
+  if (c->pcg_graph && c->pcg_graph_iters == iters) return ONSAS_OK;
+  if (c->pcg_graph) {
+    hipGraphExecDestroy(c->pcg_graph);
+    c->pcg_graph = nullptr;
+  }
+  hipStream_t s = c->stream;
+  ONSAS_HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (e != hipSuccess) return set_err(c, ONSAS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->pcg_graph, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_err(c, ONSAS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  c->pcg_graph_iters = iters;
+  return ONSAS_OK;
+}
+
+template <typename KT>
+int pcg_solve_t(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
+                int* breakdown) {
+  hipStream_t s = c->stream;
+  const int np = c->np;
+  const double rtol = c->lin.rtol > 0 ? c->lin.rtol
+                                      : (c->precision == ONSAS_PRECISION_FP64 ? 1.4901161193847656e-08 : 1e-6);
+  const int maxiter = c->lin.maxiter > 0 ? c->lin.maxiter : (int)std::min<int64_t>(c->nfree, 1 << 30);
+  int chunk_iters = c->lin.check_every > 0 ? c->lin.check_every : 16;
+  chunk_iters += chunk_iters & 1;  // even: the parity ping-pong repeats per chunk
+  hipLaunchKernelGGL(k_pcg_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
+                     (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz,
+                     part(c, kPartRZ0), part(c, kPartRR), part(c, kPartBB), np, c->chunk, c->nn);
+  hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(NT), 0, s, part(c, kPartBB), &c->d_sc->pcg,
+                     rtol, maxiter, np);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
+  // First chunk: direct launches with every pcg_A bracketed by events on this
+  // stream (the sampled duration of the SpMV kernel); then graph replays.
+  const int nsample = std::min<int>(chunk_iters, (int)(sizeof(c->ev_spmv) / sizeof(hipEvent_t)) / 2) & ~1;
+  for (int it = 0; it < nsample; ++it)
+    launch_pcg_iteration<KT>(c, it & 1, c->ev_spmv[2 * it], c->ev_spmv[2 * it + 1]);
+  for (int it = nsample; it < chunk_iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  bool first = true;
+  for (;;) {
+    if (!first) ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph, s));
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(&c->h_sc->pcg, &c->d_sc->pcg, sizeof(PcgState),
+                                    hipMemcpyDeviceToHost, s));
+    ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
+    const PcgState& ps = c->h_sc->pcg;
+    if (first) {
+      // iterations of the sampled chunk that did real work
+      const int worked = ps.breakdown ? ps.breakdown_k + 1 : (ps.done ? ps.iters : ps.k);
+      for (int it = 0; it < std::min(nsample, worked); ++it) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev_spmv[2 * it], c->ev_spmv[2 * it + 1]) == hipSuccess) {
+          c->tm.spmv_ms += ms;
+          c->tm.spmv_launches++;
+        }
+      }
+      first = false;
+    }
+    if (ps.done || ps.breakdown) break;
+  }
+  const PcgState& ps = c->h_sc->pcg;
+  *iters = ps.breakdown ? ps.breakdown_k : ps.iters;
+  *converged = ps.converged;
+  *breakdown = ps.breakdown;
+  *relres = ps.bb > 0 ? std::sqrt(ps.rr_last / ps.bb) : 0.0;
+  return ONSAS_OK;
+}
+
+template <typename KT>
+int update_t(Ctx* c) {
+  hipLaunchKernelGGL(k_update<KT>, dim3(c->np), dim3(NT), 0, c->stream, (const KT*)c->d_px,
+                     c->d_fixmask, c->d_U, part(c, kPartDU2), part(c, kPartU2), c->np, c->chunk, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+int launch_residual(Ctx* c) {
+  hipLaunchKernelGGL(k_residual, dim3(c->np), dim3(NT), 0, c->stream, c->d_fext, c->d_fint,
+                     c->d_fixmask, c->d_res, part(c, kPartR2), part(c, kPartF2), c->np, c->chunk,
+                     c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_pcg_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
+                     int* breakdown) {
+  return c->precision == ONSAS_PRECISION_FP64
+             ? pcg_solve_t<double>(c, b_dev, iters, relres, converged, breakdown)
+             : pcg_solve_t<float>(c, b_dev, iters, relres, converged, breakdown);
+}
+
+int launch_update(Ctx* c) {
+  return c->precision == ONSAS_PRECISION_FP64 ? update_t<double>(c) : update_t<float>(c);
+}
+
+int launch_finalize_newton(Ctx* c) {
+  hipLaunchKernelGGL(k_finalize_newton, dim3(1), dim3(NT), 0, c->stream, c->d_part, c->d_sc, c->np);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_matvec(Ctx* c, const double* x_dev, double* y_dev) {
+  const unsigned grid = grid_for(c->nn * kRowGroup, 256);
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_spmv_f64<double>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       c->d_col, (const double*)c->d_K, x_dev, y_dev, c->nn);
+  else
+    hipLaunchKernelGGL(k_spmv_f64<float>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       c->d_col, (const float*)c->d_K, x_dev, y_dev, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_x_to_double(Ctx* c, double* out_dev) {
+  const int64_t n = 3 * c->nn;
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_to_double<double>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
+                       (const double*)c->d_px, c->d_fixmask, out_dev, n);
+  else
+    hipLaunchKernelGGL(k_to_double<float>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
+                       (const float*)c->d_px, c->d_fixmask, out_dev, n);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_convert_K(Ctx* c, double* out_dev) {
+  const int64_t n = 9 * c->nnzb;
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_convert<double>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
+                       (const double*)c->d_K, out_dev, n);
+  else
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
+                       (const float*)c->d_K, out_dev, n);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_gmres_solve(Ctx* c, const double*, int* iters, double* relres, int* converged) {
+  *iters = 0;
+  *relres = 1.0;
+  *converged = 0;
+  return set_err(c, ONSAS_ERR_UNSUPPORTED, "GMRES fallback not available");
+}
+
+}  // namespace onsas

@@ -1,0 +1,332 @@
+"""ctypes binding of libonsas_hip.so (include/onsas_hip.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+the shared object is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libonsas_hip.so")
+
+# status codes (include/onsas_hip.h)
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NEGATIVE_VOLUME, ERR_STATE = -1, -2, -3, -4
+ERR_NOT_FINITE, ERR_LINEAR_SOLVE, ERR_ZERO_NORM, ERR_ALLOC = -5, -6, -7, -8
+ERR_INJECTED, ERR_UNSUPPORTED = -9, -10
+
+PRECISION_MIXED, PRECISION_FP64 = 0, 1
+MAT_SVK, MAT_NEOHOOKEAN, MAT_LINEAR = 1, 2, 3
+STRAIN_ROTATED, STRAIN_GREEN = 0, 1
+LINSOLVE_AUTO, LINSOLVE_PCG, LINSOLVE_GMRES = 0, 1, 2
+
+# every symbol declared in include/onsas_hip.h
+EXPORTED = (
+    "onsas_create", "onsas_destroy", "onsas_last_error", "onsas_version", "onsas_set_nodes",
+    "onsas_add_tet_group", "onsas_add_truss_group", "onsas_add_pressure_faces",
+    "onsas_set_fixed_dofs", "onsas_finalize", "onsas_set_linear_solver", "onsas_set_loads",
+    "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_set_U", "onsas_get_fext",
+    "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
+    "onsas_tangent_matvec", "onsas_element_forces", "onsas_get_stress_strain",
+    "onsas_linear_solve", "onsas_adjoint_shear", "onsas_get_timings", "onsas_reset_timings",
+    "onsas_inject_error",
+)
+
+
+class LinearSettings(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("maxiter", C.c_int32), ("rtol", C.c_double),
+                ("gmres_restart", C.c_int32), ("check_every", C.c_int32)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("dU_norm", C.c_double), ("dU_rel", C.c_double), ("r_norm", C.c_double),
+                ("r_rel", C.c_double), ("fext_norm", C.c_double), ("U_norm", C.c_double),
+                ("lin_rel_residual", C.c_double), ("lin_iters", C.c_int32),
+                ("lin_solver", C.c_int32), ("lin_converged", C.c_int32), ("reserved", C.c_int32),
+                ("assemble_ms", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Timings(C.Structure):
+    _fields_ = [("assemble_ms", C.c_double), ("scatter_ms", C.c_double), ("solve_ms", C.c_double),
+                ("pcg_ms", C.c_double), ("n_assemble", C.c_int64), ("n_steps", C.c_int64),
+                ("pcg_iters", C.c_int64), ("gmres_iters", C.c_int64),
+                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OnsasError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+
+def load():
+    """Load libonsas_hip.so (raises if it is absent: there is no CPU path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OnsasError(ERR_STATE, f"{LIB_PATH} is missing; build it with onsas.jl_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "onsas_create": ([C.c_int32, C.POINTER(_vp)], C.c_int),
+        "onsas_destroy": ([_vp], None),
+        "onsas_last_error": ([_vp], C.c_char_p),
+        "onsas_version": ([], C.c_char_p),
+        "onsas_set_nodes": ([_vp, C.c_int64, _vp, _vp, _vp], C.c_int),
+        "onsas_add_tet_group": ([_vp, C.c_int64, _vp, C.c_int32, _vp, _vp, _i32p], C.c_int),
+        "onsas_add_truss_group": ([_vp, C.c_int64, _vp, C.c_int32, C.c_double, _vp, _i32p],
+                                  C.c_int),
+        "onsas_add_pressure_faces": ([_vp, C.c_int64, _vp, _i32p], C.c_int),
+        "onsas_set_fixed_dofs": ([_vp, C.c_int64, _vp], C.c_int),
+        "onsas_finalize": ([_vp], C.c_int),
+        "onsas_set_linear_solver": ([_vp, C.POINTER(LinearSettings)], C.c_int),
+        "onsas_set_loads": ([_vp, _vp, C.c_int32, _vp], C.c_int),
+        "onsas_assemble": ([_vp, _vp], C.c_int),
+        "onsas_newton_step": ([_vp, C.POINTER(StepStats)], C.c_int),
+        "onsas_get_U": ([_vp, _vp], C.c_int),
+        "onsas_set_U": ([_vp, _vp], C.c_int),
+        "onsas_get_fext": ([_vp, _vp], C.c_int),
+        "onsas_get_fint": ([_vp, _vp], C.c_int),
+        "onsas_get_sizes": ([_vp, _i64p, _i64p, _i64p], C.c_int),
+        "onsas_get_pattern": ([_vp, _vp, _vp], C.c_int),
+        "onsas_get_tangent": ([_vp, _vp], C.c_int),
+        "onsas_tangent_matvec": ([_vp, _vp, _vp], C.c_int),
+        "onsas_element_forces": ([_vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
+        "onsas_get_stress_strain": ([_vp, C.c_int32, C.c_int32, _vp, _vp], C.c_int),
+        "onsas_linear_solve": ([_vp, _vp, _vp, _i32p, _dp], C.c_int),
+        "onsas_adjoint_shear": ([_vp, C.c_int32, _vp, _vp, _i32p], C.c_int),
+        "onsas_get_timings": ([_vp, C.POINTER(Timings)], C.c_int),
+        "onsas_reset_timings": ([_vp], C.c_int),
+        "onsas_inject_error": ([_vp, C.c_int32], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+
+
+class Context:
+    """Owner of one onsas_ctx (one HIP stream and its device memory)."""
+
+    def __init__(self, precision=PRECISION_MIXED):
+        self.L = load()
+        h = _vp()
+        st = self.L.onsas_create(int(precision), C.byref(h))
+        if st != OK:
+            raise OnsasError(st, self.L.onsas_last_error(None).decode())
+        self.h = h
+        self.precision = precision
+        self.nn = 0
+        self.ngroups = 0
+        self.group_sizes = []
+        self.group_nodes = []
+        self.npressure = 0
+
+    # -- plumbing
+    def _check(self, st):
+        if st != OK:
+            raise OnsasError(st, self.L.onsas_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.onsas_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- model
+    def set_nodes(self, coords):
+        X = np.ascontiguousarray(np.asarray(coords, dtype=np.float64).reshape(-1, 3))
+        x, y, z = (np.ascontiguousarray(X[:, d]) for d in range(3))
+        self._check(self.L.onsas_set_nodes(self.h, X.shape[0], x.ctypes.data, y.ctypes.data,
+                                           z.ctypes.data))
+        self.nn = X.shape[0]
+
+    def add_tet_group(self, conn, material, params, shear_per_elem=None):
+        c = _i32(conn)
+        p = _f64(params, 2)
+        g = _f64(shear_per_elem) if shear_per_elem is not None else None
+        gid = C.c_int32()
+        self._check(self.L.onsas_add_tet_group(self.h, c.size // 4, c.ctypes.data, int(material),
+                                               p.ctypes.data, _ptr(g), C.byref(gid)))
+        self.group_sizes.append(c.size // 4)
+        self.group_nodes.append(4)
+        return gid.value
+
+    def add_truss_group(self, conn, strain_model, E, area):
+        c = _i32(conn)
+        ne = c.size // 2
+        a = np.broadcast_to(np.asarray(area, dtype=np.float64), (ne,)).copy()
+        gid = C.c_int32()
+        self._check(self.L.onsas_add_truss_group(self.h, ne, c.ctypes.data, int(strain_model),
+                                                 float(E), a.ctypes.data, C.byref(gid)))
+        self.group_sizes.append(ne)
+        self.group_nodes.append(2)
+        return gid.value
+
+    def add_pressure_faces(self, conn):
+        c = _i32(conn)
+        gid = C.c_int32()
+        self._check(self.L.onsas_add_pressure_faces(self.h, c.size // 3, c.ctypes.data,
+                                                    C.byref(gid)))
+        self.npressure += 1
+        return gid.value
+
+    def set_fixed_dofs(self, dofs):
+        d = np.ascontiguousarray(dofs, dtype=np.int64).reshape(-1)
+        self._check(self.L.onsas_set_fixed_dofs(self.h, d.size, d.ctypes.data))
+
+    def finalize(self):
+        self._check(self.L.onsas_finalize(self.h))
+
+    def set_linear_solver(self, kind=LINSOLVE_AUTO, rtol=0.0, maxiter=0, gmres_restart=40,
+                          check_every=16):
+        s = LinearSettings(int(kind), int(maxiter), float(rtol), int(gmres_restart),
+                           int(check_every))
+        self._check(self.L.onsas_set_linear_solver(self.h, C.byref(s)))
+
+    # -- per step / iteration
+    def set_loads(self, nodal=None, pressures=()):
+        nd = None if nodal is None else _f64(nodal, 3 * self.nn)
+        p = _f64(pressures) if len(pressures) else None
+        self._check(self.L.onsas_set_loads(self.h, _ptr(nd), 0 if p is None else p.size, _ptr(p)))
+
+    def assemble(self, want_fint=False):
+        f = np.empty(3 * self.nn) if want_fint else None
+        self._check(self.L.onsas_assemble(self.h, _ptr(f)))
+        return f
+
+    def newton_step(self):
+        s = StepStats()
+        self._check(self.L.onsas_newton_step(self.h, C.byref(s)))
+        return s
+
+    # -- state
+    def _get(self, fn, n):
+        out = np.empty(n)
+        self._check(fn(self.h, out.ctypes.data))
+        return out
+
+    def get_U(self):
+        return self._get(self.L.onsas_get_U, 3 * self.nn)
+
+    def set_U(self, U):
+        u = _f64(U, 3 * self.nn)
+        self._check(self.L.onsas_set_U(self.h, u.ctypes.data))
+
+    def get_fext(self):
+        return self._get(self.L.onsas_get_fext, 3 * self.nn)
+
+    def get_fint(self):
+        return self._get(self.L.onsas_get_fint, 3 * self.nn)
+
+    def sizes(self):
+        nn, nnzb, nc = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.L.onsas_get_sizes(self.h, C.byref(nn), C.byref(nnzb), C.byref(nc)))
+        return nn.value, nnzb.value, nc.value
+
+    def pattern(self):
+        nn, nnzb, _ = self.sizes()
+        rp = np.empty(nn + 1, dtype=np.int32)
+        col = np.empty(nnzb, dtype=np.int32)
+        self._check(self.L.onsas_get_pattern(self.h, rp.ctypes.data, col.ctypes.data))
+        return rp, col
+
+    def tangent_bsr(self):
+        """The assembled tangent as a scipy BSR matrix (3x3 blocks, FP64 copy)."""
+        import scipy.sparse as sp
+        rp, col = self.pattern()
+        vals = np.empty(9 * col.size)
+        self._check(self.L.onsas_get_tangent(self.h, vals.ctypes.data))
+        n = 3 * (rp.size - 1)
+        return sp.bsr_matrix((vals.reshape(-1, 3, 3), col, rp), shape=(n, n))
+
+    def matvec(self, x):
+        xv = _f64(x, 3 * self.nn)
+        y = np.empty_like(xv)
+        self._check(self.L.onsas_tangent_matvec(self.h, xv.ctypes.data, y.ctypes.data))
+        return y
+
+    def element_forces(self, group, want_K=True):
+        ne, k = self.group_sizes[group], self.group_nodes[group]
+        nd = 3 * k
+        f = np.empty(ne * nd)
+        K = np.empty(ne * nd * nd) if want_K else None
+        s = np.empty(ne * 9)
+        e = np.empty(ne * 9)
+        self._check(self.L.onsas_element_forces(self.h, group, f.ctypes.data, _ptr(K),
+                                                s.ctypes.data, e.ctypes.data))
+        return (f.reshape(ne, nd), None if K is None else K.reshape(ne, nd, nd),
+                s.reshape(ne, 3, 3), e.reshape(ne, 3, 3))
+
+    def stress_strain(self, group, at_last_assembly=True):
+        ne = self.group_sizes[group]
+        s = np.empty(ne * 9)
+        e = np.empty(ne * 9)
+        self._check(self.L.onsas_get_stress_strain(self.h, group, int(bool(at_last_assembly)),
+                                                   s.ctypes.data, e.ctypes.data))
+        return s.reshape(ne, 3, 3), e.reshape(ne, 3, 3)
+
+    def linear_solve(self, b):
+        bv = _f64(b, 3 * self.nn)
+        x = np.empty_like(bv)
+        it, rr = C.c_int32(), C.c_double()
+        self._check(self.L.onsas_linear_solve(self.h, bv.ctypes.data, x.ctypes.data, C.byref(it),
+                                              C.byref(rr)))
+        return x, it.value, rr.value
+
+    def adjoint_shear(self, group, dJdU):
+        b = _f64(dJdU, 3 * self.nn)
+        out = np.empty(self.group_sizes[group])
+        it = C.c_int32()
+        self._check(self.L.onsas_adjoint_shear(self.h, group, b.ctypes.data, out.ctypes.data,
+                                               C.byref(it)))
+        return out, it.value
+
+    def timings(self):
+        t = Timings()
+        self._check(self.L.onsas_get_timings(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def reset_timings(self):
+        self._check(self.L.onsas_reset_timings(self.h))
+
+    def inject_error(self, code):
+        self._check(self.L.onsas_inject_error(self.h, int(code)))

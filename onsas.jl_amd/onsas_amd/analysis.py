@@ -1,0 +1,332 @@
+"""Nonlinear static analysis, Newton-Raphson and the solve entry points.
+
+ConvergenceSettings, criteria, ResidualsIterationStep, isconverged! --
+    src/StructuralSolvers/StructuralSolvers.jl:36-171
+NewtonRaphson -- src/StructuralSolvers/Solvers.jl:14-26
+NonLinearStaticAnalysis, _solve!, step! --
+    src/StructuralAnalyses/NonLinearStaticAnalyses.jl:32-148
+solve / solve! -- src/StructuralSolvers/StructuralSolvers.jl:201-222
+
+Julia's `name!` functions are spelled `name_` here (solve_, step_, assemble_,
+isconverged_, reset_, update_, next_).  The host runs exactly the reference's
+load-step / Newton control flow; every array operation of the loop body runs in
+libonsas_hip through MI355XBackend.
+"""
+from __future__ import annotations
+
+import copy
+import time
+import warnings
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .backend import GpuStaticState, MI355XBackend
+
+INITIAL_DELTA = 1e12  # StructuralSolvers.jl:27
+
+
+# ---- convergence (StructuralSolvers.jl:36-171) ----------------------------------
+
+@dataclass(frozen=True)
+class ConvergenceSettings:
+    """ConvergenceSettings(rel_U_tol, rel_res_force_tol, max_iter) (StructuralSolvers.jl:36-43)."""
+    rel_U_tol: float = 1e-6
+    rel_res_force_tol: float = 1e-6
+    max_iter: int = 20
+
+
+def residual_forces_tol(x):
+    if isinstance(x, ConvergenceSettings):
+        return x.rel_res_force_tol
+    return (x.dr_rel, x.dr_norm)
+
+
+def displacement_tol(x):
+    if isinstance(x, ConvergenceSettings):
+        return x.rel_U_tol
+    return (x.dU_rel, x.dU_norm)
+
+
+def max_iter_tol(cs):
+    return cs.max_iter
+
+
+class AbstractConvergenceCriterion:
+    def __repr__(self):
+        return type(self).__name__ + "()"
+
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self))
+
+
+class ResidualForceCriterion(AbstractConvergenceCriterion):
+    pass
+
+
+class ΔUCriterion(AbstractConvergenceCriterion):
+    pass
+
+
+DeltaUCriterion = ΔUCriterion
+
+
+class ΔU_and_ResidualForce_Criteria(AbstractConvergenceCriterion):
+    pass
+
+
+class MaxIterCriterion(AbstractConvergenceCriterion):
+    pass
+
+
+class NotConvergedYet(AbstractConvergenceCriterion):
+    pass
+
+
+@dataclass
+class ResidualsIterationStep:
+    """Convergence record of the current iteration (StructuralSolvers.jl:85-97)."""
+    dU_norm: float = INITIAL_DELTA
+    dr_norm: float = INITIAL_DELTA
+    dU_rel: float = INITIAL_DELTA
+    dr_rel: float = INITIAL_DELTA
+    iter: int = 0
+    criterion: AbstractConvergenceCriterion = field(default_factory=NotConvergedYet)
+
+
+def iterations(x):
+    return x.iter if isinstance(x, ResidualsIterationStep) else x.iterations()
+
+
+def criterion(x):
+    return x.criterion if isinstance(x, ResidualsIterationStep) else x.criteria()
+
+
+def reset_(ri: ResidualsIterationStep):
+    """reset! (StructuralSolvers.jl:115-121)."""
+    ri.dU_norm = ri.dr_norm = ri.dU_rel = ri.dr_rel = INITIAL_DELTA
+    ri.iter = 0
+    ri.criterion = NotConvergedYet()
+    return ri
+
+
+def update_(ri: ResidualsIterationStep, *args):
+    """update!(ri, criterion) or update!(ri, dU_norm, dU_rel, dr_norm, dr_rel)
+    (StructuralSolvers.jl:131-145)."""
+    if len(args) == 1:
+        ri.criterion = args[0]
+        return ri
+    ri.dU_norm, ri.dU_rel, ri.dr_norm, ri.dr_rel = (float(a) for a in args)
+    ri.iter += 1
+    return ri
+
+
+def isconverged_(ri: ResidualsIterationStep, cs: ConvergenceSettings):
+    """isconverged! (StructuralSolvers.jl:148-171): residual criterion first, then
+    dU, then the iteration cap (warns), else not converged."""
+    dU_rel, _ = displacement_tol(ri)
+    dr_rel, _ = residual_forces_tol(ri)
+    if not dU_rel > 0:
+        raise AssertionError("Residual displacements norm must be greater than 0.")
+    if not dr_rel > 0:
+        raise AssertionError("Residual forces norm must be greater than 0.")
+    if dr_rel <= cs.rel_res_force_tol:
+        c = ResidualForceCriterion()
+    elif dU_rel <= cs.rel_U_tol:
+        c = ΔUCriterion()
+    elif ri.iter > cs.max_iter:
+        warnings.warn("Maximum number of iterations was reached.")
+        c = MaxIterCriterion()
+    else:
+        c = NotConvergedYet()
+    update_(ri, c)
+    return c
+
+
+# ---- solvers (Solvers.jl:14-26) -------------------------------------------------
+
+class NewtonRaphson:
+    def __init__(self, tol: ConvergenceSettings = None):
+        self.tol = tol if tol is not None else ConvergenceSettings()
+
+
+def tolerances(alg):
+    return alg.tol
+
+
+# ---- analysis (NonLinearStaticAnalyses.jl:32-60) ---------------------------------
+
+class NonLinearStaticAnalysis:
+    """NonLinearStaticAnalysis(s, t1=1.0; NSTEPS=10, initial_step=1) with load
+    factors LinRange(t1/NSTEPS, t1, NSTEPS), or NonLinearStaticAnalysis(s,
+    load_factors=...) (NonLinearStaticAnalyses.jl:45-60)."""
+
+    def __init__(self, s, t1=1.0, NSTEPS=10, initial_step=1, load_factors=None,
+                 initial_state=None):
+        if load_factors is None:
+            t0 = t1 / NSTEPS
+            load_factors = np.linspace(t0, t1, NSTEPS)
+        self.s = s
+        self.load_factors = np.asarray(load_factors, dtype=np.float64)
+        if not 1 <= initial_step <= len(self.load_factors):
+            raise ValueError(f"initial_step must be in [1, {len(self.load_factors)}] but is: "
+                             f"{initial_step}.")
+        self.current_step = int(initial_step)
+        self.initial_state = initial_state  # optional U to resume from
+        self.state = None                   # GpuStaticState, created by solve_
+        self.iter_state = ResidualsIterationStep()
+
+    def is_done(self):
+        """is_done (StaticAnalyses.jl:70-77)."""
+        if self.current_step > len(self.load_factors):
+            self.current_step -= 1
+            return True
+        return False
+
+    def current_time(self):
+        return float(self.load_factors[self.current_step - 1])
+
+    def next_(self):
+        self.current_step += 1
+
+
+def structure(sa):
+    return sa.s
+
+
+def load_factors(sa):
+    return sa.load_factors
+
+
+def current_iteration(sa):
+    return sa.iter_state
+
+
+def current_state(sa):
+    return sa.state
+
+
+# ---- solution (src/StructuralSolvers/Solutions.jl) --------------------------------
+
+class Solution:
+    """Per-load-step states: U, per-group stress/strain (at the last assembly
+    of the step, like store!, StaticAnalyses.jl:157-174) and the convergence
+    record, plus the engine's linear-solver statistics."""
+
+    def __init__(self, sa, alg):
+        self.analysis = sa
+        self.algorithm = alg
+        self.U = [None] * len(sa.load_factors)
+        self.stress = [None] * len(sa.load_factors)
+        self.strain = [None] * len(sa.load_factors)
+        self.records = [None] * len(sa.load_factors)
+        self.newton_log = []  # one dict per Newton iteration
+        self.wall_s = 0.0
+
+    def displacements(self, step=None, node=None, component=None):
+        U = self.U if step is None else [self.U[step]]
+        out = np.array([u for u in U])
+        if node is not None:
+            out = out[:, 3 * node: 3 * node + 3]
+            if component is not None:
+                out = out[:, component - 1]
+        return out if step is None else out[0]
+
+    def criteria(self):
+        return [r["criterion"] if r else None for r in self.records]
+
+    def iterations(self):
+        return [r["iterations"] if r else None for r in self.records]
+
+    def linear_iterations(self):
+        return sum(d["lin_iters"] for d in self.newton_log)
+
+    def table(self):
+        """Rows of _print_table (Solutions.jl:106-134)."""
+        rows = []
+        for k, r in enumerate(self.records):
+            if r is None:
+                continue
+            rows.append((k + 1, np.linalg.norm(self.U[k]), r["dU_rel"], r["dr_norm"], r["dr_rel"],
+                         r["criterion"], r["iterations"]))
+        return rows
+
+
+def store_(sol: Solution, sa: NonLinearStaticAnalysis, step: int):
+    st = sa.state
+    sol.U[step - 1] = st.displacements()
+    if st.backend.store_stresses:
+        ss = st.stress_strain(at_last_assembly=True)
+        sol.stress[step - 1] = [s for _, s, _ in ss]
+        sol.strain[step - 1] = [e for _, _, e in ss]
+    ri = sa.iter_state
+    sol.records[step - 1] = dict(dU_norm=ri.dU_norm, dU_rel=ri.dU_rel, dr_norm=ri.dr_norm,
+                                 dr_rel=ri.dr_rel, iterations=ri.iter, criterion=ri.criterion)
+
+
+# ---- the hot path ---------------------------------------------------------------
+
+def assemble_(s, sa):
+    """assemble!(s, sa) (StaticAnalyses.jl:99-122): f_int and K at the current U, on the GPU."""
+    sa.state.ctx.assemble()
+
+
+def step_(sa, alg, linear_solver=None, sol=None):
+    """step!(sa, ::NewtonRaphson, linear_solver) (NonLinearStaticAnalyses.jl:107-148):
+    r = f_ext - f_int, solve K dU = r, the four norms, U[free] += dU, update!."""
+    stats = sa.state.ctx.newton_step()
+    update_(sa.iter_state, stats.dU_norm, stats.dU_rel, stats.r_norm, stats.r_rel)
+    if sol is not None:
+        d = stats.as_dict()
+        d["load_step"] = sa.current_step
+        d["iteration"] = sa.iter_state.iter
+        sol.newton_log.append(d)
+    return stats
+
+
+def _solve_nonlinear_(sa, alg, backend):
+    """_solve!(::NonLinearStaticAnalysis, alg, linear_solver)
+    (NonLinearStaticAnalyses.jl:70-104) with the MI355X engine."""
+    s = sa.s
+    if sa.state is None:
+        sa.state = GpuStaticState(s, backend)
+    if sa.initial_state is not None:
+        sa.state.ctx.set_U(sa.initial_state)
+    sol = Solution(sa, alg)
+    t0 = time.perf_counter()
+    while not sa.is_done():
+        step = sa.current_step
+        reset_(sa.iter_state)
+        sa.state.apply_loads(sa.current_time())
+        while isinstance(isconverged_(sa.iter_state, tolerances(alg)), NotConvergedYet):
+            assemble_(s, sa)
+            step_(sa, alg, backend, sol)
+        store_(sol, sa, step)
+        sa.next_()
+    sol.wall_s = time.perf_counter() - t0
+    return sol
+
+
+def solve_(problem, solver=None, linear_solve=None):
+    """solve!(problem, solver, linear_solve) (StructuralSolvers.jl:217-222)."""
+    if linear_solve is None:
+        linear_solve = MI355XBackend()
+    if not isinstance(linear_solve, MI355XBackend):
+        raise TypeError("the engine path needs linear_solve = MI355XBackend(...)")
+    if isinstance(problem, NonLinearStaticAnalysis):
+        return _solve_nonlinear_(problem, solver or NewtonRaphson(), linear_solve)
+    from .linear_analysis import LinearStaticAnalysis, _solve_linear_
+    if isinstance(problem, LinearStaticAnalysis):
+        return _solve_linear_(problem, linear_solve)
+    raise TypeError(f"unsupported problem {type(problem)}")
+
+
+def solve(problem, solver=None, linear_solve=None):
+    """solve(problem, ...) = solve!(deepcopy(problem), ...) (StructuralSolvers.jl:201-206)."""
+    p = copy.copy(problem)
+    p.iter_state = ResidualsIterationStep()
+    p.state = None
+    return solve_(p, solver, linear_solve)

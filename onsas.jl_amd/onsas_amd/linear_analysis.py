@@ -1,0 +1,73 @@
+"""LinearStaticAnalysis on the engine (src/StructuralAnalyses/LinearStaticAnalyses.jl:
+25-153): K assembled at the first load step, U = K^-1 f_ext per step (same
+block-Jacobi PCG), a re-assembly per step for the stresses and strains.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .analysis import ΔU_and_ResidualForce_Criteria, ResidualsIterationStep, Solution, store_
+from .backend import GpuStaticState
+
+
+class LinearStaticAnalysis:
+    """LinearStaticAnalysis(s, final_time=1.0; NSTEPS=10, initial_step=1) or with
+    load_factors (LinearStaticAnalyses.jl:48-68)."""
+
+    def __init__(self, s, final_time=1.0, NSTEPS=10, initial_step=1, load_factors=None):
+        if load_factors is None:
+            load_factors = np.linspace(final_time / NSTEPS, final_time, NSTEPS)
+        self.s = s
+        self.load_factors = np.asarray(load_factors, dtype=np.float64)
+        if not 1 <= initial_step <= len(self.load_factors):
+            raise ValueError(f"initial_step must be in [1, {len(self.load_factors)}] but is: "
+                             f"{initial_step}.")
+        self.current_step = int(initial_step)
+        self.state = None
+        # LinearResidualsIterationStep (LinearStaticAnalyses.jl:26-28)
+        self.iter_state = ResidualsIterationStep(None, None, None, None, 1,
+                                                 ΔU_and_ResidualForce_Criteria())
+        self.initial_state = None
+
+    def is_done(self):
+        if self.current_step > len(self.load_factors):
+            self.current_step -= 1
+            return True
+        return False
+
+    def current_time(self):
+        return float(self.load_factors[self.current_step - 1])
+
+    def next_(self):
+        self.current_step += 1
+
+
+def _solve_linear_(sa: LinearStaticAnalysis, backend):
+    """_solve!(::LinearStaticAnalysis, nothing, linear_solver) (LinearStaticAnalyses.jl:78-114)."""
+    if sa.state is None:
+        sa.state = GpuStaticState(sa.s, backend)
+    st = sa.state
+    ctx = st.ctx
+    sol = Solution(sa, None)
+    t0 = time.perf_counter()
+    while not sa.is_done():
+        step = sa.current_step
+        st.apply_loads(sa.current_time())
+        if step == 1:
+            ctx.assemble()
+        # step!: res = f_ext[free]; solve K dU = res; U[free] .= dU (:117-153)
+        fext = ctx.get_fext()
+        b = np.zeros_like(fext)
+        b[st.free_dofs] = fext[st.free_dofs]
+        x, it, rr = ctx.linear_solve(b)
+        U = np.zeros_like(fext)
+        U[st.free_dofs] = x[st.free_dofs]
+        ctx.set_U(U)
+        sol.newton_log.append({"load_step": step, "lin_iters": it, "lin_rel_residual": rr})
+        ctx.assemble()
+        store_(sol, sa, step)
+        sa.next_()
+    sol.wall_s = time.perf_counter() - t0
+    return sol

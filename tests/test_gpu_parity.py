@@ -1,0 +1,315 @@
+"""GPU parity: the HIP engine (called through the C ABI) against the FP64 oracle
+and the reference examples' closed forms.
+
+Tolerances: FP64 quantities (f_int, residual, U, element outputs) to ~1e-12
+relative; the FP32 tangent of the mixed mode to 2e-6 of the largest entry of
+its row block; converged displacements to 1e-8 relative of max|U| when the
+Newton residual is driven below 1e-10 (both precisions iterate to the same FP64
+residual, so the converged states must agree).
+"""
+import numpy as np
+import pytest
+
+from _helpers import fext_steps
+from onsas_amd import _capi, meshes, workloads
+from onsas_amd.analysis import ConvergenceSettings, NewtonRaphson, NonLinearStaticAnalysis, solve
+from onsas_amd.backend import MI355XBackend
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"mixed": _capi.PRECISION_MIXED, "fp64": _capi.PRECISION_FP64}
+MATS = {"svk": (_capi.MAT_SVK, (0.6e6, 0.4e6)), "nh": (_capi.MAT_NEOHOOKEAN, (1.3e6, 0.4e6)),
+        "linear": (_capi.MAT_LINEAR, (1.0e6, 0.3))}
+
+
+def _ctx(prec, coords, tets=None, mat="svk", trusses=None, strain=0, E=1.0, A=None, fixed=(),
+         faces=None, G=None):
+    c = _capi.Context(PREC[prec])
+    c.set_nodes(coords)
+    if tets is not None:
+        code, p = MATS[mat]
+        c.add_tet_group(tets, code, p, G)
+    if trusses is not None:
+        c.add_truss_group(trusses, strain, E, A)
+    if faces is not None:
+        c.add_pressure_faces(faces)
+    c.set_fixed_dofs(np.asarray(fixed, dtype=np.int64))
+    c.finalize()
+    return c
+
+
+def _oracle_model(oracle, coords, tets, mat, fixed=(), G=None):
+    code = {"svk": oracle.MAT_SVK, "nh": oracle.MAT_NEOHOOKEAN, "linear": oracle.MAT_LINEAR}[mat]
+    return oracle.Model(coords, tets=tets, tet_mat=code, tet_p=MATS[mat][1], tet_G=G,
+                        fixed_dofs=fixed)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("mat", ["svk", "nh", "linear"])
+def test_tet_element_outputs(oracle, prec, mat):
+    """Batch internal_forces on the GPU vs the oracle, element by element."""
+    print(f"[gpu] tet element outputs {mat} {prec}", flush=True)
+    sm = meshes.cube(2, jitter=0.1, seed=7)
+    rng = np.random.default_rng(11)
+    U = rng.normal(scale=0.04, size=3 * sm.nn)
+    c = _ctx(prec, sm.coords, sm.tets, mat)
+    c.set_U(U)
+    f, K, s, e = c.element_forces(0)
+    code, p = MATS[mat]
+    for k, tet in enumerate(sm.tets):
+        X = sm.coords[tet].reshape(-1)
+        u = U.reshape(-1, 3)[tet].reshape(-1)
+        if mat == "linear":
+            fo, Ko, so, eo = oracle.tet_linear(p[0], p[1], X, u)
+        else:
+            fo, Ko, so, eo = oracle.tet_hyperelastic({"svk": oracle.MAT_SVK, "nh": oracle.MAT_NEOHOOKEAN}[mat],
+                                                     p[0], p[1], X, u)
+        np.testing.assert_allclose(f[k], fo, rtol=1e-11, atol=1e-11 * np.abs(fo).max())
+        np.testing.assert_allclose(K[k], Ko, rtol=1e-11, atol=1e-11 * np.abs(Ko).max())
+        np.testing.assert_allclose(s[k], so, rtol=1e-11, atol=1e-11 * np.abs(so).max())
+        np.testing.assert_allclose(e[k], eo, rtol=1e-12, atol=1e-14)
+    c.close()
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_truss_element_outputs(oracle, model):
+    print(f"[gpu] truss element outputs model={model}", flush=True)
+    tl = meshes.truss_lattice(2, h=0.5, jitter=0.05)
+    rng = np.random.default_rng(3)
+    U = rng.normal(scale=0.01, size=3 * tl.nn)
+    A = rng.uniform(0.5, 1.5, size=len(tl.trusses))
+    c = _ctx("fp64", tl.coords, trusses=tl.trusses, strain=model, E=2.0, A=A)
+    c.set_U(U)
+    f, K, s, e = c.element_forces(0)
+    for k, (a, b) in enumerate(tl.trusses):
+        d = np.r_[3 * a:3 * a + 3, 3 * b:3 * b + 3]
+        fo, Ko, so, eo = oracle.truss(model, 2.0, A[k], tl.coords[[a, b]].reshape(-1), U[d])
+        np.testing.assert_allclose(f[k], fo, rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(K[k], Ko, rtol=1e-10, atol=1e-12 * np.abs(Ko).max())
+        assert s[k][0, 0] == pytest.approx(so, rel=1e-11, abs=1e-15)
+        assert e[k][0, 0] == pytest.approx(eo, rel=1e-11, abs=1e-15)
+    c.close()
+
+
+def _bsr_vs_oracle(Kg, Ko, fixed, n, tol):
+    """Compare the engine's tangent (identity on fixed rows/cols) with the
+    oracle's full tangent restricted to free dofs."""
+    Kg = Kg.toarray()
+    Ko = Ko.toarray()
+    free = np.setdiff1d(np.arange(n), fixed)
+    scale = np.abs(Ko).max()
+    np.testing.assert_allclose(Kg[np.ix_(free, free)], Ko[np.ix_(free, free)], rtol=0,
+                               atol=tol * scale)
+    np.testing.assert_array_equal(Kg[np.ix_(fixed, fixed)], np.eye(len(fixed)))
+    assert np.abs(Kg[np.ix_(fixed, free)]).max(initial=0) == 0
+    assert np.abs(Kg[np.ix_(free, fixed)]).max(initial=0) == 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("mat", ["svk", "nh", "linear"])
+def test_assembly_matches_oracle(oracle, prec, mat):
+    """Coloured block-CSR assembly of f_int and K vs the oracle's COO->CSR assembly."""
+    print(f"[gpu] assembly {mat} {prec}", flush=True)
+    s, sm = workloads.cube_extension(n=4, jitter=0.1, seed=3)
+    fixed = s.fixed_dofs
+    rng = np.random.default_rng(5)
+    U = rng.normal(scale=0.02, size=3 * sm.nn)
+    U[fixed] = 0
+    c = _ctx(prec, sm.coords, sm.tets, mat, fixed=fixed)
+    c.set_U(U)
+    fint = c.assemble(want_fint=True)
+    Kg = c.tangent_bsr()
+    out = _oracle_model(oracle, sm.coords, sm.tets, mat, fixed).assemble(U)
+    np.testing.assert_allclose(fint, out["fint"], rtol=0, atol=1e-12 * np.abs(out["fint"]).max())
+    _bsr_vs_oracle(Kg, out["K"], fixed, 3 * sm.nn, 1e-12 if prec == "fp64" else 2e-6)
+    # y = K x through the C ABI
+    x = rng.normal(size=3 * sm.nn)
+    y = c.matvec(x)
+    xf = x.copy()
+    xf[fixed] = 0
+    yo = out["K"] @ xf
+    yo[fixed] = x[fixed]
+    np.testing.assert_allclose(y, yo, rtol=0, atol=(1e-11 if prec == "fp64" else 3e-6) * np.abs(yo).max())
+    c.close()
+
+
+def test_assembly_is_deterministic():
+    """Two assemblies of the same state give the same bits (no float atomics)."""
+    print("[gpu] assembly determinism", flush=True)
+    sm = meshes.cube(8, jitter=0.1, seed=9)
+    U = np.random.default_rng(2).normal(scale=0.01, size=3 * sm.nn)
+    for prec in ("mixed", "fp64"):
+        c = _ctx(prec, sm.coords, sm.tets, "nh")
+        c.set_U(U)
+        f1 = c.assemble(True)
+        K1 = c.tangent_bsr().data.copy()
+        f2 = c.assemble(True)
+        K2 = c.tangent_bsr().data.copy()
+        assert np.array_equal(f1, f2) and np.array_equal(K1, K2)
+        c.close()
+
+
+def test_pressure_loads_match_oracle(oracle):
+    print("[gpu] pressure loads", flush=True)
+    sm = meshes.cube(3, jitter=0.1, seed=1)
+    faces = np.concatenate([sm.faces["x1"], sm.faces["y1"], sm.faces["z0"]])
+    c = _ctx("fp64", sm.coords, sm.tets, "svk", faces=faces)
+    nodal = np.random.default_rng(0).normal(size=3 * sm.nn)
+    c.set_loads(nodal, [2.5])
+    f = c.get_fext()
+    ref = nodal.copy()
+    for tri in faces:
+        fe = oracle.face_pressure(sm.coords[tri].reshape(-1), 2.5)
+        for a in range(3):
+            ref[3 * tri[a]:3 * tri[a] + 3] += fe[a]
+    np.testing.assert_allclose(f, ref, rtol=1e-14, atol=1e-15)
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pcg_solve_matches_direct(oracle, prec):
+    """Block-Jacobi PCG on the assembled tangent vs a sparse direct solve."""
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] pcg vs direct {prec}", flush=True)
+    s, sm = workloads.cube_extension(n=6, jitter=0.1, seed=4)
+    fixed = s.fixed_dofs
+    c = _ctx(prec, sm.coords, sm.tets, "svk", fixed=fixed)
+    c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=1e-12 if prec == "fp64" else 1e-6, maxiter=5000)
+    c.assemble()
+    b = np.random.default_rng(1).normal(size=3 * sm.nn)
+    x, it, rr = c.linear_solve(b)
+    out = _oracle_model(oracle, sm.coords, sm.tets, "svk", fixed).assemble(np.zeros(3 * sm.nn),
+                                                                         want_full=False,
+                                                                         want_ff=True)
+    free = np.setdiff1d(np.arange(3 * sm.nn), fixed)
+    xo = spla.spsolve(out["Kff"].tocsc(), b[free])
+    err = np.abs(x[free] - xo).max() / np.abs(xo).max()
+    assert err < (1e-9 if prec == "fp64" else 2e-4), err
+    assert np.all(x[fixed] == 0)
+    assert 0 < it < 5000
+    c.close()
+
+
+def _newton(structure, nsteps, prec, tol_f=1e-10, **kw):
+    sa = NonLinearStaticAnalysis(structure, NSTEPS=nsteps)
+    alg = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-14, rel_res_force_tol=tol_f, max_iter=30))
+    be = MI355XBackend(precision=prec, maxiter=20000, **kw)
+    return solve(sa, alg, be)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_uniaxial_extension_cube(prec):
+    """SVK cube 4^3 x 6 tets, tension 3e5 Pa, 10 steps, tol 1e-10: every load
+    step is the homogeneous closed-form state (uniaxial_extension.jl:151-205)."""
+    print(f"[gpu] uniaxial extension {prec}", flush=True)
+    s, sm = workloads.cube_extension(n=4)
+    sol = _newton(s, 10, prec)
+    for k, t in enumerate(np.linspace(0.1, 1.0, 10)):
+        a = workloads.svk_uniaxial_alpha(1e6, 3e5 * t)
+        b = workloads.svk_uniaxial_beta(a, 0.3)
+        ex = workloads.homogeneous_displacement(sm.coords, a, b)
+        np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-9 * np.abs(ex).max())
+        assert sol.records[k]["dr_rel"] <= 1e-10
+        assert type(sol.records[k]["criterion"]).__name__ == "ResidualForceCriterion"
+    # stress at the converged state: P11 = p, C = F'F
+    P = sol.stress[-1][0]
+    a = workloads.svk_uniaxial_alpha(1e6, 3e5)
+    np.testing.assert_allclose(P[:, 0, 0], 3e5, rtol=1e-6)
+    np.testing.assert_allclose(sol.strain[-1][0][:, 0, 0], a * a, rtol=1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_nh_compression_vs_oracle(oracle, prec):
+    """NeoHookean jittered cube in compression: per-step displacements equal
+    the oracle's Newton (reference algorithm) and the closed form."""
+    print(f"[gpu] NH compression vs oracle {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=4, jitter=0.05, seed=2)
+    nsteps = 5
+    sol = _newton(s, nsteps, prec)
+    lfs = np.linspace(1 / nsteps, 1.0, nsteps)
+    m = oracle.Model(sm.coords, tets=sm.tets, tet_mat=oracle.MAT_NEOHOOKEAN, tet_p=(8.3e5, 3.8e5),
+                     fixed_dofs=s.fixed_dofs)
+    _, Us, _ = m.newton(fext_steps(oracle, s, lfs), rel_U_tol=1e-14, rel_res_tol=1e-10, max_iter=30)
+    for k in range(nsteps):
+        scale = np.abs(Us[k]).max()
+        np.testing.assert_allclose(sol.U[k], Us[k], rtol=0, atol=1e-8 * scale)
+        a = workloads.nh_uniaxial_alpha(-p * lfs[k], 8.3e5, 3.8e5)
+        b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+        ex = workloads.homogeneous_displacement(sm.coords, a, b)
+        np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-8 * scale)
+
+
+def test_trusses_green_clamped(golden):
+    """Clamped Green-strain bar (examples/clamped_truss) through the API on the GPU."""
+    print("[gpu] clamped truss", flush=True)
+    from onsas_amd import (FixedField, GlobalLoad, Mesh, NodeSet, SVK, Square, StructuralBoundaryCondition,
+                           StructuralMaterial, Structure, Trusses, GreenStrain)
+    g = golden["clamped_truss"]
+    N, E, L, A, F = g["N"], g["E"], g["L"], g["A"], g["F"]
+    x = np.linspace(0, L, N + 1)
+    coords = np.stack([x, 0 * x, 0 * x], 1)
+    bars = Trusses(np.stack([np.arange(N), np.arange(1, N + 1)], 1), Square(np.sqrt(A)), GreenStrain)
+    s = Structure(Mesh(coords, [bars]), StructuralMaterial({SVK(E=E, nu=g["nu"]): bars}),
+                  StructuralBoundaryCondition({
+                      FixedField("u", [1]): [NodeSet([0])],
+                      FixedField("u", [2, 3]): [NodeSet(np.arange(N + 1))],
+                      GlobalLoad("u", lambda t: [F * t, 0, 0]): [NodeSet([N])]}))
+    sa = NonLinearStaticAnalysis(s, NSTEPS=g["NSTEPS"])
+    for prec in ("fp64", "mixed"):
+        sol = solve(sa, NewtonRaphson(), MI355XBackend(precision=prec))
+        u = np.array([U[3 * N] for U in sol.U])
+        eg = 0.5 * ((L + u) ** 2 - L ** 2) / L ** 2
+        P = (L + u) / L * E * eg * A
+        np.testing.assert_allclose(P, F * sa.load_factors, rtol=g["rtol"])
+
+
+def test_error_paths():
+    """Inverted tet -> ERR_NEGATIVE_VOLUME; call order -> ERR_STATE; injected
+    error -> its code; an indefinite tangent breaks PCG down (ERR_LINEAR_SOLVE)."""
+    print("[gpu] error paths", flush=True)
+    X = np.array([[0, 0, 0], [0, 1, 0], [0, 0, 1], [2, 0, 1.0]])
+    c = _capi.Context(_capi.PRECISION_FP64)
+    c.set_nodes(X)
+    c.add_tet_group([[1, 0, 2, 3]], _capi.MAT_SVK, (1.0, 1.0))
+    with pytest.raises(_capi.OnsasError) as e:
+        c.finalize()
+    assert e.value.code == _capi.ERR_NEGATIVE_VOLUME
+    c.close()
+    c = _ctx("fp64", X, [[0, 1, 2, 3]], "svk", fixed=[0, 1, 2, 3, 4, 5, 6, 7, 8])
+    with pytest.raises(_capi.OnsasError) as e:
+        c.newton_step()
+    assert e.value.code == _capi.ERR_STATE
+    c.inject_error(_capi.ERR_INJECTED)
+    with pytest.raises(_capi.OnsasError) as e:
+        c.assemble()
+    assert e.value.code == _capi.ERR_INJECTED
+    c.assemble()
+    c.close()
+    # a straight two-bar chain compressed axially: negative transverse stiffness
+    X = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0.0]])
+    c = _ctx("fp64", X, trusses=[[0, 1], [1, 2]], E=1.0, A=[1.0, 1.0], fixed=[0, 1, 2, 6, 7, 8])
+    c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=1e-10, maxiter=100)
+    c.set_U(np.array([0, 0, 0, 0.0, 0, 0, -0.2, 0, 0]))
+    c.assemble()
+    with pytest.raises(_capi.OnsasError) as e:
+        c.linear_solve(np.array([0, 0, 0, 0, 1.0, 0, 0, 0, 0]))
+    assert e.value.code == _capi.ERR_LINEAR_SOLVE
+    c.close()
+
+
+def test_full_size_compression_cube_mixed():
+    """The 64^3 NeoHookean compression cube (1,572,864 tets) in mixed precision:
+    the converged state is homogeneous, so every node must match the closed
+    form to 1e-6 relative at full load (size-independent property)."""
+    print("[gpu] 64^3 compression cube, mixed, 20 steps", flush=True)
+    s, sm, p = workloads.cube_compression(n=64)
+    sol = _newton(s, 20, "mixed", tol_f=1e-8, store_stresses=False)
+    a = workloads.nh_uniaxial_alpha(-p, 8.3e5, 3.8e5)
+    assert a == pytest.approx(0.85, abs=1e-9)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    ex = workloads.homogeneous_displacement(sm.coords, a, b)
+    err = np.abs(sol.U[-1] - ex).max() / np.abs(ex).max()
+    print(f"[gpu] 64^3 max rel error {err:.3e}, wall {sol.wall_s:.2f}s, "
+          f"lin iters {sol.linear_iterations()}", flush=True)
+    assert err <= 1e-6
+    assert all(r["dr_rel"] <= 1e-8 for r in sol.records)

@@ -1,0 +1,261 @@
+"""Pins of the oracle that the reference's golden vectors leave open: what the
+mathematics fixes (energy/tangent consistency by central differences,
+objectivity, symmetry, patch test) and the closed-form answers of the
+reference's own examples (uniaxial extension / compression, von Mises truss,
+clamped truss, Lame cylinder).  Each is chosen so that a dropped term, a wrong
+sign or a transposed operand in the oracle fails it."""
+import numpy as np
+import pytest
+
+from _helpers import fext_steps, oracle_model
+from onsas_amd import meshes, workloads
+
+RNG = np.random.default_rng(1234)
+X_TET = np.array([0, 0, 0, 0, 1, 0, 0, 0, 1, 2, 0, 1.0])
+
+
+def _rand_E(scale=0.1):
+    A = RNG.normal(scale=scale, size=(3, 3))
+    return 0.5 * (A + A.T)
+
+
+def _fd_stress(energy, E, h=1e-6):
+    """dPsi/dE with symmetric perturbations (E_ij and E_ji together)."""
+    S = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(i, 3):
+            D = np.zeros((3, 3))
+            D[i, j] = D[j, i] = h
+            d = (energy(E + D) - energy(E - D)) / (2 * h)
+            S[i, j] = S[j, i] = d if i == j else d / 2
+    return S
+
+
+@pytest.mark.parametrize("mat", ["svk", "nh"])
+def test_stress_is_energy_gradient(oracle, mat):
+    """S = dPsi/dE (SVKMaterial.jl:57-60, NeoHookeanMaterial.jl:59-66)."""
+    lam, G, K = 0.7, 0.4, 1.1
+    for _ in range(5):
+        E = _rand_E()
+        if mat == "svk":
+            S, _ = oracle.svk_stress(lam, G, E)
+            fd = _fd_stress(lambda e: oracle.svk_energy(lam, G, e), E)
+        else:
+            S, _ = oracle.nh_stress(K, G, E)
+            fd = _fd_stress(lambda e: oracle.nh_energy(K, G, e), E)
+        np.testing.assert_allclose(S, fd, rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("mat", ["svk", "nh"])
+def test_dSdE_is_stress_derivative(oracle, mat):
+    """Voigt dS/dE (engineering-shear columns, factor 0.5 of
+    NeoHookeanMaterial.jl:122-127) against central differences of S."""
+    lam, G, K = 0.7, 0.4, 1.1
+    f = (lambda e: oracle.svk_stress(lam, G, e)) if mat == "svk" else \
+        (lambda e: oracle.nh_stress(K, G, e))
+    VI = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for _ in range(5):
+        E = _rand_E()
+        _, D = f(E)
+        h = 1e-6
+        for q, (k, l) in enumerate(VI):
+            P = np.zeros((3, 3))
+            P[k, l] = P[l, k] = h / (1 if k == l else 2)  # engineering shear increment h
+            dS = (f(E + P)[0] - f(E - P)[0]) / (2 * h)
+            col = np.array([dS[i, j] for i, j in VI])
+            np.testing.assert_allclose(D[:, q], col, rtol=1e-6, atol=1e-8)
+
+
+def _tet_case(oracle, kind):
+    if kind == "svk":
+        return lambda X, u: oracle.tet_hyperelastic(oracle.MAT_SVK, 0.6, 0.4, X, u)
+    if kind == "nh":
+        return lambda X, u: oracle.tet_hyperelastic(oracle.MAT_NEOHOOKEAN, 1.3, 0.4, X, u)
+    return lambda X, u: oracle.tet_linear(2.0, 0.3, X, u)
+
+
+@pytest.mark.parametrize("kind", ["svk", "nh", "linear"])
+def test_tet_tangent_is_force_jacobian(oracle, kind):
+    """K = df/du by central differences (Tetrahedrons.jl:186-266)."""
+    ev = _tet_case(oracle, kind)
+    u = RNG.normal(scale=0.05, size=12)
+    f, K, _, _ = ev(X_TET, u)
+    h = 1e-6
+    J = np.zeros((12, 12))
+    for c in range(12):
+        e = np.zeros(12)
+        e[c] = h
+        J[:, c] = (ev(X_TET, u + e)[0] - ev(X_TET, u - e)[0]) / (2 * h)
+    np.testing.assert_allclose(K, J, rtol=1e-6, atol=1e-8)
+    np.testing.assert_array_equal(K, K.T)
+
+
+@pytest.mark.parametrize("kind", ["svk", "nh"])
+def test_tet_force_is_energy_gradient_and_objective(oracle, kind):
+    """f = d(V Psi(E(u)))/du, zero under a finite rigid rotation, and K
+    annihilates rigid translations at any state."""
+    ev = _tet_case(oracle, kind)
+    dN, V = oracle.tet_gradients(X_TET)
+    energy = (lambda E: oracle.svk_energy(0.6, 0.4, E)) if kind == "svk" else \
+        (lambda E: oracle.nh_energy(1.3, 0.4, E))
+
+    def W(u):
+        H = u.reshape(4, 3).T @ dN
+        return V * energy(0.5 * (H + H.T + H.T @ H))
+
+    u = RNG.normal(scale=0.05, size=12)
+    f, K, _, _ = ev(X_TET, u)
+    h = 1e-6
+    g = np.array([(W(u + h * e) - W(u - h * e)) / (2 * h) for e in np.eye(12)])
+    np.testing.assert_allclose(f, g, rtol=1e-6, atol=1e-9)
+    t = np.tile([0.3, -0.2, 0.5], 4)
+    np.testing.assert_allclose(K @ t, 0, atol=1e-12)
+    c, s = np.cos(0.7), np.sin(0.7)
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    X = X_TET.reshape(4, 3)
+    ur = (X @ R.T - X).reshape(-1)
+    fr, _, _, _ = ev(X_TET, ur)
+    np.testing.assert_allclose(fr, 0, atol=1e-12)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_truss_tangent_is_force_jacobian(oracle, model):
+    X = np.array([0.1, -0.2, 0.3, 1.1, 0.4, -0.2])
+    u = RNG.normal(scale=0.05, size=6)
+    f, K, _, _ = oracle.truss(model, 2.0, 0.3, X, u)
+    h = 1e-7
+    J = np.zeros((6, 6))
+    for c in range(6):
+        e = np.zeros(6)
+        e[c] = h
+        J[:, c] = (oracle.truss(model, 2.0, 0.3, X, u + e)[0] -
+                   oracle.truss(model, 2.0, 0.3, X, u - e)[0]) / (2 * h)
+    np.testing.assert_allclose(K, J, rtol=1e-6, atol=1e-8)
+
+
+def test_patch_test_jittered_mesh(oracle):
+    """Affine u on a jittered mesh: uniform stress, so f_int vanishes at every
+    interior node (assembly + kernel, any element shape)."""
+    sm = meshes.cube(3, jitter=0.15, seed=5)
+    A = np.array([[0.02, 0.01, -0.03], [0.0, -0.01, 0.02], [0.015, 0.0, 0.01]])
+    U = (sm.coords @ A.T).reshape(-1)
+    interior = np.setdiff1d(np.arange(sm.nn), np.concatenate(
+        [sm.node_sets[k] for k in ("x0", "x1", "y0", "y1", "z0", "z1")]))
+    for mat, p in ((oracle.MAT_SVK, (0.6, 0.4)), (oracle.MAT_NEOHOOKEAN, (1.3, 0.4)),
+                   (oracle.MAT_LINEAR, (2.0, 0.3))):
+        m = oracle.Model(sm.coords, tets=sm.tets, tet_mat=mat, tet_p=p)
+        f = m.assemble(U, want_full=False)["fint"].reshape(-1, 3)
+        assert np.abs(f[interior]).max() < 1e-13
+        assert np.abs(f).max() > 1e-4
+
+
+def test_nh_uniaxial_closed_form_on_a_tet(oracle):
+    """Homogeneous uniaxial state F = diag(a, b, b): P11 and P22 of the
+    oracle's NeoHookean equal uniaxial_compression.jl:245-255."""
+    K, G = 8.3e5, 3.8e5
+    for a in (0.85, 0.95, 1.1):
+        b = workloads.nh_uniaxial_beta(a, K, G)
+        F = np.diag([a, b, b])
+        E = 0.5 * (F.T @ F - np.eye(3))
+        S, _ = oracle.nh_stress(K, G, E)
+        P = F @ S
+        assert P[0, 0] == pytest.approx(workloads.nh_uniaxial_P11(a, K, G), rel=1e-12)
+        assert abs(P[1, 1]) < 1e-9 * K
+
+
+def test_uniaxial_extension_closed_form(oracle):
+    """Cube in tension (uniaxial_extension.jl:151-205): the converged FE state is
+    exactly homogeneous; alpha, beta match the SVK closed form per load step."""
+    s, sm = workloads.cube_extension(n=2, E=1e6, nu=0.3, p=3e5)
+    lfs = np.linspace(0.1, 1.0, 10)
+    m = oracle_model(oracle, s)
+    U, Us, recs = m.newton(fext_steps(oracle, s, lfs), rel_U_tol=1e-14, rel_res_tol=1e-10, max_iter=30)
+    for k, t in enumerate(lfs):
+        a = workloads.svk_uniaxial_alpha(1e6, 3e5 * t)
+        b = workloads.svk_uniaxial_beta(a, 0.3)
+        np.testing.assert_allclose(Us[k], workloads.homogeneous_displacement(sm.coords, a, b),
+                                   rtol=0, atol=1e-9)
+        assert recs[k]["criterion"] == 1
+    assert workloads.svk_uniaxial_alpha(1e6, 3e5) == pytest.approx(1.2211967, abs=1e-7)
+
+
+def test_von_mises_truss(oracle, golden):
+    """examples/von_misses_truss/von_misses_truss.jl: load factor vs analytic
+    lambda(u_k) for both strain models, rtol 1e-4."""
+    g = golden["von_mises_truss"]
+    E, A, L, Fk = g["E"], g["A"], g["L"], g["Fk"]
+    th = np.deg2rad(g["theta_deg"])
+    V, H = L * np.cos(th), L * np.sin(th)
+    X = np.array([0, 0, 0, V, 0, H, 2 * V, 0, 0.0])
+    fixed = [0, 1, 2, 6, 7, 8, 4]
+    lfs = np.linspace(1 / g["NSTEPS"], 1.0, g["NSTEPS"])
+    for model in (0, 1):
+        m = oracle.Model(X, trusses=[[0, 1], [1, 2]], truss_strain=model, truss_E=E,
+                         truss_A=[A, A], fixed_dofs=fixed)
+        F = np.zeros((len(lfs), 9))
+        F[:, 5] = Fk * lfs
+        _, Us, _ = m.newton(F, rel_U_tol=g["tol_u"], rel_res_tol=g["tol_f"],
+                            max_iter=g["max_iter"])
+        uk = Us[:, 5]
+        if model == 0:
+            lam = -2 * E * A * ((H + uk) ** 2 + V ** 2 - L ** 2) / (
+                L * (L + np.sqrt((H + uk) ** 2 + V ** 2))) * (H + uk) / np.sqrt((H + uk) ** 2 + V ** 2)
+        else:
+            lam = -2 * E * A * ((H + uk) * (2 * H * uk + uk ** 2)) / (2.0 * L ** 3)
+        np.testing.assert_allclose(lam, -lfs * Fk, rtol=g["rtol"])
+        assert np.abs(Us[:, 3]).max() <= 100 * np.finfo(float).eps
+
+
+def test_clamped_truss(oracle, golden):
+    """examples/clamped_truss/clamped_truss.jl: Green-strain bar, tip force vs
+    analytic P(u), rtol 1e-3 (1-D bar embedded in 3-D with y, z fixed)."""
+    g = golden["clamped_truss"]
+    N, E, L, A, F = g["N"], g["E"], g["L"], g["A"], g["F"]
+    x = np.linspace(0, L, N + 1)
+    X = np.stack([x, np.zeros_like(x), np.zeros_like(x)], 1).reshape(-1)
+    conn = np.stack([np.arange(N), np.arange(1, N + 1)], 1)
+    fixed = [0] + [3 * i + d for i in range(N + 1) for d in (1, 2)]
+    m = oracle.Model(X, trusses=conn, truss_strain=1, truss_E=E, truss_A=np.full(N, A),
+                     fixed_dofs=fixed)
+    lfs = np.linspace(0.1, 1.0, g["NSTEPS"])
+    Fs = np.zeros((len(lfs), 3 * (N + 1)))
+    Fs[:, 3 * N] = F * lfs
+    _, Us, _ = m.newton(Fs)
+    u = Us[:, 3 * N]
+    eg = 0.5 * ((L + u) ** 2 - L ** 2) / L ** 2
+    P = (L + u) / L * E * eg * A
+    np.testing.assert_allclose(P, F * lfs, rtol=g["rtol"])
+
+
+def test_cylinder_lame(oracle):
+    """examples/cylinder_internal_pressure: linear thick cylinder vs the
+    plane-strain Lame solution.  Linear tets converge to it under refinement;
+    the reference's own tolerance is ATOL = 1e-2 (Re - Ri) (11 % of u_r(Ri))."""
+    ex_i = workloads.lame_ur(0.1, 0.1, 0.2, 210e9, 0.3, 1e7)
+    ex_o = workloads.lame_ur(0.2, 0.1, 0.2, 210e9, 0.3, 1e7)
+    assert ex_i == pytest.approx(9.0794e-6, rel=1e-4) and ex_o == pytest.approx(5.7778e-6, rel=1e-4)
+    errs = []
+    for nr, nt in ((4, 32), (8, 64)):
+        s, sm = workloads.thick_cylinder(nr=nr, ntheta=nt, nz=2, Lz=0.05)
+        m = oracle_model(oracle, s)
+        _, Us, _ = m.newton(fext_steps(oracle, s, [1.0]), rel_U_tol=1e-12, rel_res_tol=1e-10)
+        U = Us[0].reshape(-1, 3)
+        r = np.linalg.norm(sm.coords[:, :2], axis=1)
+        ur = (U[:, 0] * sm.coords[:, 0] + U[:, 1] * sm.coords[:, 1]) / r
+        e = max(np.abs(ur[sm.node_sets["inner"]] / ex_i - 1).max(),
+                np.abs(ur[sm.node_sets["outer"]] / ex_o - 1).max())
+        errs.append(e)
+        assert np.abs(U[:, 2]).max() < 0.01 * ex_o  # ends fixed in z: near plane strain
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 0.035
+
+
+def test_newton_criteria_semantics(oracle):
+    """isconverged! ordering: residual criterion first, and the recorded
+    residual is that of U before the last update (one extra dU is applied)."""
+    s, sm = workloads.cube_extension(n=1)
+    m = oracle_model(oracle, s)
+    F = fext_steps(oracle, s, [1.0])
+    _, _, rec = m.newton(F, rel_U_tol=1e-14, rel_res_tol=1e-10, max_iter=30)
+    assert rec[0]["criterion"] == 1 and rec[0]["r_rel"] <= 1e-10
+    _, _, rec = m.newton(F, rel_U_tol=1e-14, rel_res_tol=1e-10, max_iter=0)
+    assert rec[0]["criterion"] == 3 and rec[0]["iters"] == 1
