@@ -62,6 +62,12 @@ typedef struct {
                               mode, 1e-6 in mixed mode                                  */
     int32_t gmres_restart; /* Krylov subspace dimension (<= 0: 40)                      */
     int32_t check_every;   /* PCG iterations per device-resident chunk (<= 0: 16)       */
+    int32_t forcing;       /* 1: inexact-Newton forcing terms (Eisenstat-Walker choice 2
+                              with a "just enough" floor), 0: fixed rtol                 */
+    double newton_tol;     /* forcing: the Newton rel_res_force_tol                      */
+    double eta_max;        /* forcing: loosest inner tolerance (<= 0: 0.1)               */
+    double eta_first;      /* forcing: inner tolerance of the first Newton iteration of a
+                              load step (<= 0: 1e-3)                                      */
 } onsas_linear_settings;
 
 /* Per-Newton-iteration record: the four norms of
@@ -80,6 +86,7 @@ typedef struct {
     int32_t reserved;
     double assemble_ms;      /* device time of the preceding onsas_assemble          */
     double solve_ms;         /* device time of this step's residual+solve+update     */
+    double lin_rtol;         /* inner relative tolerance used by this step            */
 } onsas_step_stats;
 
 /* Cumulative device-time counters (hipEvents), reset by onsas_reset_timings. */

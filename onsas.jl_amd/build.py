@@ -47,6 +47,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # bandwidth probe (a separate executable, not part of the library)
+    probe = os.path.join(OBJ, "stream_probe")
+    r = subprocess.run([HIPCC, *FLAGS, os.path.join(CSRC, "stream_probe.hip"), "-o", probe],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for stream_probe.hip:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -71,15 +71,15 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 // Solve K x = b_dev with the configured linear solver; x left in the PCG x buffer (KT).
-int linear_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
-                 int* solver) {
+int linear_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
+                 int* converged, int* solver) {
   int it = 0, conv = 0, brk = 0;
   double rr = 1.0;
   *solver = 0;
   *iters = 0;
   const int kind = c->lin.kind;
   if (kind == ONSAS_LINSOLVE_PCG || kind == ONSAS_LINSOLVE_AUTO) {
-    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, &it, &rr, &conv, &brk));
+    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, forcing, &it, &rr, &conv, &brk));
     *solver = 1;
     *iters = it;
     c->tm.pcg_iters += it;
@@ -153,7 +153,7 @@ void onsas_destroy(onsas_ctx* ctx) {
                   c->d_Dinv, c->d_U, c->d_Uasm, c->d_fint, c->d_fext, c->d_res, c->d_nodal, c->d_px, c->d_pr,
                   c->d_pz, c->d_pp0, c->d_pp1, c->d_pq, c->d_part, c->d_sc, c->d_face_conn,
                   c->d_face_grp, c->d_nf_ptr, c->d_nf_idx, c->d_pressures, c->d_gm_V, c->d_gm_w,
-                  c->d_gm_h};
+                  c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -291,6 +291,7 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
     ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_pressures, pressures, npressure * sizeof(double),
                                     hipMemcpyHostToDevice, c->stream));
   ONSAS_TRY(onsas::launch_fext(c));
+  ONSAS_TRY(onsas::reset_forcing(c));  // a new load step: no previous inner residual
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return ONSAS_OK;
 }
@@ -302,6 +303,7 @@ int onsas_assemble(onsas_ctx* ctx, double* fint_out) {
   ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_Uasm, c->d_U, 3 * c->nn * sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
   ONSAS_TRY(onsas::launch_assemble(c));
+  ONSAS_TRY(onsas::launch_pack_sell(c));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   if (fint_out)
     ONSAS_HIP_TRY(c, hipMemcpyAsync(fint_out, c->d_fint, 3 * c->nn * sizeof(double),
@@ -327,7 +329,7 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
   int iters = 0, conv = 0, solver = 0;
   double relres = 0;
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[5], s));
-  ONSAS_TRY(linear_solve(c, c->d_res, &iters, &relres, &conv, &solver));
+  ONSAS_TRY(linear_solve(c, c->d_res, true, &iters, &relres, &conv, &solver));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[6], s));
   ONSAS_TRY(onsas::launch_update(c));
   ONSAS_TRY(onsas::launch_finalize_newton(c));
@@ -346,6 +348,7 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
   out->lin_solver = solver;
   out->lin_converged = conv;
   out->lin_rel_residual = relres;
+  out->lin_rtol = (c->lin.forcing && solver == 1) ? nw.eta : onsas::linear_rtol(c);
   out->assemble_ms = c->last_assemble_ms;
   out->solve_ms = elapsed(c->ev[4], c->ev[7]);
   c->tm.solve_ms += out->solve_ms;
@@ -491,7 +494,7 @@ int onsas_linear_solve(onsas_ctx* ctx, const double* b, double* x, int32_t* iter
   ONSAS_TRY(to_device(c, dx, nullptr, n));
   int it = 0, conv = 0, solver = 0;
   double rr = 0;
-  ONSAS_TRY(linear_solve(c, db.p, &it, &rr, &conv, &solver));
+  ONSAS_TRY(linear_solve(c, db.p, false, &it, &rr, &conv, &solver));
   ONSAS_TRY(onsas::launch_x_to_double(c, dx.p));
   ONSAS_HIP_TRY(c, hipMemcpyAsync(x, dx.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -516,7 +519,7 @@ int onsas_adjoint_shear(onsas_ctx* ctx, int32_t group, const double* dJdU, doubl
   ONSAS_TRY(to_device(c, dout, nullptr, g.ne));
   int it = 0, conv = 0, solver = 0;
   double rr = 0;
-  ONSAS_TRY(linear_solve(c, db.p, &it, &rr, &conv, &solver));
+  ONSAS_TRY(linear_solve(c, db.p, false, &it, &rr, &conv, &solver));
   if (!conv) return set_err(c, ONSAS_ERR_LINEAR_SOLVE, "adjoint solve did not reach its tolerance");
   ONSAS_TRY(onsas::launch_adjoint_shear(c, group, c->d_px, dout.p));
   ONSAS_HIP_TRY(c, hipMemcpyAsync(dJdG, dout.p, g.ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));

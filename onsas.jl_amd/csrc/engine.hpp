@@ -23,8 +23,24 @@ namespace onsas {
 
 constexpr int kNodesTet = 4;
 constexpr int kNodesTruss = 2;
-constexpr int kRedThreads = 1024;  // threads per workgroup of every reduction-producing kernel
-constexpr int kRowGroup = 16;      // lanes per block row in the SpMV kernels
+constexpr int kRedThreads = 512;   // threads per workgroup of every reduction-producing kernel
+constexpr int kRowGroup = 16;      // lanes per block row in the block-CSR SpMV (matvec API)
+#ifndef ONSAS_SELL_LANES
+#define ONSAS_SELL_LANES 1
+#endif
+#ifndef ONSAS_SPMV_THREADS
+#define ONSAS_SPMV_THREADS 512
+#endif
+#ifndef ONSAS_SPMV_WAVES_PER_EU
+#define ONSAS_SPMV_WAVES_PER_EU 1
+#endif
+#ifndef ONSAS_SPMV_UNROLL
+#define ONSAS_SPMV_UNROLL 1
+#endif
+constexpr int kSellLanes = ONSAS_SELL_LANES;     // lanes per block row in the sliced-ELL SpMV
+constexpr int kSellRows = 64 / kSellLanes;       // block rows per slice (one wave per slice)
+constexpr int kSpmvThreads = ONSAS_SPMV_THREADS; // threads per workgroup of the sliced-ELL SpMV
+constexpr int kChunkRows = (kSpmvThreads / 64) * kSellRows;  // rows per workgroup chunk unit
 constexpr int kMaxPartials = 1024; // upper bound of the reduction grid (multiple of 8)
 
 enum GroupType { kTet = 0, kTruss = 1 };
@@ -69,6 +85,8 @@ struct NewtonStats {  // device-resident FP64 norms of one Newton step
   double fext2;       // ||f_ext||^2
   double dU2;         // ||dU||^2
   double U2;          // ||U||^2 before the update
+  double prev_b2;     // ||b||^2 of the previous inner solve of this load step (0: none)
+  double eta;         // inner tolerance used by the last solve
 };
 
 struct Scalars {      // one allocation so a single copy returns everything
@@ -98,6 +116,24 @@ struct Ctx {
   std::vector<int32_t> row_ptr, col, diag;
   int64_t nnzb = 0;
 
+  // sliced-ELL copy of the tangent for the PCG SpMV: slices of kSellRows block
+  // rows, each slice padded to width_s slots (a multiple of kSellLanes).  Lane
+  // l = kSellRows * ph + r of the wave owning slice s reads slot k = kSellLanes m + ph
+  // of row kSellRows s + r: block index sell_ptr[s] + 64 m + l, its 9 values at
+  // 9 (sell_ptr[s] + 64 m) + 64 v + l (one 256-byte plane per value).
+  std::vector<int32_t> sell_ptr;
+  int64_t nslices = 0, nsell = 0;
+  // round-major variant (ELL): when the padding allows it, round m of every
+  // slice is stored contiguously -- block index (m * nslices + s) * 64 + lane --
+  // so the waves sweeping round m at the same time read one contiguous window.
+  bool ell_rounds = false;
+  int32_t ell_width = 0;              // rounds of the widest slice
+  std::vector<int32_t> slice_rounds;  // rounds per slice
+  int32_t* d_slice_rounds = nullptr;
+  int32_t* d_sell_ptr = nullptr;
+  int32_t* d_sell_col = nullptr;
+  void* d_Ksell = nullptr;
+
   // device
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
   uint8_t* d_fixmask = nullptr;
@@ -119,7 +155,7 @@ struct Ctx {
 
   // reduction grid
   int np = 8;          // workgroups of reduction kernels (multiple of 8)
-  int64_t chunk = 1;   // rows (nodes) per logical workgroup
+  int64_t chunk = 1;   // rows (nodes) per logical workgroup, a multiple of kChunkRows
 
   // linear solver settings
   onsas_linear_settings lin{};
@@ -161,12 +197,15 @@ int launch_adjoint_shear(Ctx* c, int g, const void* lambda_kt, double* dJdG_dev)
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2
-int launch_pcg_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
-                     int* breakdown);
+int launch_pcg_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
+                     int* converged, int* breakdown);
+double linear_rtol(const Ctx* c);                               // configured / default rtol
+int reset_forcing(Ctx* c);                                      // new load step
 int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged);
 int launch_update(Ctx* c);                                      // U += x, partial DU2/U2
 int launch_finalize_newton(Ctx* c);                             // partials -> d_sc->nwt
 int launch_matvec(Ctx* c, const double* x_dev, double* y_dev);
+int launch_pack_sell(Ctx* c);                                  // block-CSR -> sliced-ELL copy
 int launch_x_to_double(Ctx* c, double* out_dev);               // PCG x (KT) -> FP64 (fixed = 0)
 int launch_convert_K(Ctx* c, double* out_dev);
 

@@ -3,6 +3,7 @@
 // element->block slot maps, then the upload of everything to HBM.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -251,11 +252,57 @@ int finalize_model(Ctx* c) {
   ONSAS_TRY(dev_alloc(c, (void**)&c->d_sc, sizeof(Scalars)));
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
 
-  // reduction grid: >= 64 rows per workgroup, at most kMaxPartials/2 workgroups, multiple of 8
-  int64_t want = (nn + 63) / 64;
-  int64_t np = std::min<int64_t>(512, std::max<int64_t>(8, ((want + 7) / 8) * 8));
+  // reduction grid: one row per thread when it fits, whole 64-row slices per
+  // workgroup, a multiple of 8 workgroups (XCD mapping), at most kMaxPartials
+  int64_t want = (nn + kRedThreads - 1) / kRedThreads;
+  int64_t np = std::min<int64_t>(kMaxPartials, std::max<int64_t>(8, ((want + 7) / 8) * 8));
   c->np = (int)np;
-  c->chunk = (nn + np - 1) / np;
+  c->chunk = (((nn + np - 1) / np) + kChunkRows - 1) / kChunkRows * kChunkRows;
+
+  // sliced-ELL layout of the tangent (kSellRows block rows per slice, padded
+  // with zero blocks whose column is the row itself)
+  c->nslices = (nn + kSellRows - 1) / kSellRows;
+  c->sell_ptr.assign(c->nslices + 1, 0);
+  c->slice_rounds.assign(c->nslices, 0);
+  c->ell_width = 0;
+  for (int64_t s = 0; s < c->nslices; ++s) {
+    int32_t w = 0;
+    for (int64_t r = kSellRows * s; r < std::min<int64_t>(nn, kSellRows * s + kSellRows); ++r)
+      w = std::max(w, c->row_ptr[r + 1] - c->row_ptr[r]);
+    w = (w + kSellLanes - 1) / kSellLanes;  // slot rounds
+    c->slice_rounds[s] = w;
+    c->ell_width = std::max(c->ell_width, w);
+    c->sell_ptr[s + 1] = c->sell_ptr[s] + 64 * w;
+  }
+  // round-major when its padding costs at most 25 % over the per-slice layout
+  const char* lay = std::getenv("ONSAS_SELL_LAYOUT");
+  const int64_t nell = (int64_t)c->ell_width * c->nslices * 64;
+  c->ell_rounds = (lay == nullptr || std::string(lay) != "slices") &&
+                  nell <= c->sell_ptr[c->nslices] + c->sell_ptr[c->nslices] / 4;
+  if (c->ell_rounds)
+    for (int64_t s = 0; s <= c->nslices; ++s) c->sell_ptr[s] = (int32_t)(64 * s);
+  c->nsell = c->ell_rounds ? nell : c->sell_ptr[c->nslices];
+  const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;  // blocks between rounds
+  if (c->nsell >= (int64_t(1) << 31) / 9)
+    return set_err(c, ONSAS_ERR_UNSUPPORTED, "tangent too large for 32-bit sliced-ELL offsets");
+  {
+    std::vector<int32_t> scol(c->nsell, 0);
+    for (int64_t s = 0; s < c->nslices; ++s) {
+      const int32_t rounds = c->slice_rounds[s];
+      for (int m = 0; m < rounds; ++m)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int64_t r = kSellRows * s + lane % kSellRows;
+          const int k = kSellLanes * m + lane / kSellRows;
+          int32_t cc = 0;
+          if (r < nn) cc = (k < c->row_ptr[r + 1] - c->row_ptr[r]) ? c->col[c->row_ptr[r] + k] : (int32_t)r;
+          scol[c->sell_ptr[s] + rstride * m + lane] = cc;
+        }
+    }
+    ONSAS_TRY(dev_upload(c, &c->d_slice_rounds, c->slice_rounds.data(), c->slice_rounds.size()));
+    ONSAS_TRY(dev_upload(c, &c->d_sell_ptr, c->sell_ptr.data(), c->sell_ptr.size()));
+    ONSAS_TRY(dev_upload(c, &c->d_sell_col, scol.data(), scol.size()));
+    ONSAS_TRY(dev_alloc(c, &c->d_Ksell, c->kt * 9 * (size_t)c->nsell));
+  }
 
   for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
   for (auto& e : c->ev_spmv) ONSAS_HIP_TRY(c, hipEventCreate(&e));

@@ -19,6 +19,7 @@
 // Chunks of iterations are captured once in a HIP graph and replayed; the host
 // reads the device state between chunks.
 #include <cmath>
+#include <cstdlib>
 
 #include "engine.hpp"
 
@@ -28,6 +29,7 @@ namespace {
 
 constexpr int NT = kRedThreads;
 
+template <int T = NT>
 __device__ inline double block_sum(double v, double* sh) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -36,7 +38,7 @@ __device__ inline double block_sum(double v, double* sh) {
   __syncthreads();
   double s = 0;
 #pragma unroll
-  for (int w = 0; w < NT / 64; ++w) s += sh[w];
+  for (int w = 0; w < T / 64; ++w) s += sh[w];
   return s;
 }
 
@@ -44,6 +46,41 @@ __device__ inline double sum_parts(const double* __restrict__ p, int np, double*
   double v = 0;
   for (int i = threadIdx.x; i < np; i += NT) v += p[i];
   return block_sum(v, sh);
+}
+
+// Sums of up to three partial arrays in one pass (one L2 round trip, one LDS
+// exchange).  p2 may be null (its sum is then 0).  sh needs 3 * T / 64 doubles.
+template <int T = NT>
+__device__ inline void sum_parts3(const double* __restrict__ p0, const double* __restrict__ p1,
+                                  const double* __restrict__ p2, int np, double* sh, double& s0,
+                                  double& s1, double& s2) {
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int i = threadIdx.x; i < np; i += T) {
+    v0 += p0[i];
+    v1 += p1[i];
+    if (p2) v2 += p2[i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, 64);
+    v1 += __shfl_xor(v1, o, 64);
+    v2 += __shfl_xor(v2, o, 64);
+  }
+  constexpr int W = T / 64;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = v0;
+    sh[W + (threadIdx.x >> 6)] = v1;
+    sh[2 * W + (threadIdx.x >> 6)] = v2;
+  }
+  __syncthreads();
+  s0 = s1 = s2 = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    s0 += sh[w];
+    s1 += sh[W + w];
+    s2 += sh[2 * W + w];
+  }
 }
 
 // XCD-aware row range: consecutive hardware workgroup ids land on different
@@ -175,13 +212,40 @@ __global__ __launch_bounds__(NT) void k_pcg_init(const double* __restrict__ b,
   }
 }
 
+// Stopping rule of the inner solve.  Fixed: ||r|| <= rtol ||b||.  Forcing
+// (inexact Newton): eta = max(0.9 (||b_k|| / ||b_{k-1}||)^2 [Eisenstat-Walker
+// choice 2, gamma = 0.9, alpha = 2], 0.1 tau / ||b_k||) clamped to
+// [rtol, eta_max], where b_k is this Newton iteration's residual and
+// tau = newton_tol ||f_ext|| the outer target: no inner accuracy is bought
+// beyond what the quadratic model or the outer tolerance can use.  The first
+// iteration of a load step uses eta_first.
 __global__ __launch_bounds__(NT) void k_pcg_init_finalize(const double* __restrict__ pbb,
-                                                            PcgState* __restrict__ st, double rtol,
-                                                            int maxiter, int np) {
+                                                            const double* __restrict__ pf2,
+                                                            PcgState* __restrict__ st,
+                                                            NewtonStats* __restrict__ nw, double rtol,
+                                                            int maxiter, int np, int forcing,
+                                                            double newton_tol, double eta_max,
+                                                            double eta_first) {
   __shared__ double sh[NT / 64];
   const double bb = sum_parts(pbb, np, sh);
+  double eta = rtol;
+  if (forcing) {
+    const double ff = sum_parts(pf2, np, sh);
+    const double tau = newton_tol * sqrt(ff);
+    const double prev = nw->prev_b2;
+    double e = prev > 0 ? 0.9 * (bb / prev) : eta_first;
+    if (bb > 0) e = fmax(e, 0.1 * tau / sqrt(bb));
+    // the residual already meets the outer tolerance: this solve only refines
+    // (the reference applies it before its convergence test sees the residual)
+    const double cap = (bb > 0 && sqrt(bb) <= tau) ? fmax(eta_max, 0.5) : eta_max;
+    eta = fmin(fmax(e, rtol), cap);
+  }
   if (threadIdx.x == 0) {
-    st->tol2 = rtol * rtol * bb;
+    if (forcing) {
+      nw->prev_b2 = bb;
+      nw->eta = eta;
+    }
+    st->tol2 = eta * eta * bb;
     st->bb = bb;
     st->rr_last = bb;
     st->k = 0;
@@ -194,23 +258,64 @@ __global__ __launch_bounds__(NT) void k_pcg_init_finalize(const double* __restri
   }
 }
 
+// Block-CSR (assembly layout, AoS 3x3 blocks) -> sliced-ELL (SpMV layout):
+// one wave per 64-row slice, lane = row; slot k of row r goes to the value
+// planes 9 (sell_ptr[s] + 64 k) + 64 v + lane, so every SpMV load of a value
+// plane is one contiguous 256-byte access.  Padding slots are zero.
 template <typename KT>
-__global__ __launch_bounds__(NT) void k_pcg_A(const int32_t* __restrict__ row_ptr,
-                                                const int32_t* __restrict__ col,
-                                                const KT* __restrict__ val, const KT* __restrict__ z,
-                                                const KT* __restrict__ pold, KT* __restrict__ pnew,
-                                                KT* __restrict__ q, const double* __restrict__ prz_cur,
-                                                const double* __restrict__ prz_prev,
-                                                const double* __restrict__ prr,
-                                                double* __restrict__ ppq, PcgState* __restrict__ st,
-                                                int np, int64_t chunk, int64_t nn) {
-  __shared__ double sh[NT / 64];
+__global__ __launch_bounds__(256) void k_pack_sell(const int32_t* __restrict__ row_ptr,
+                                                     const KT* __restrict__ K,
+                                                     const int32_t* __restrict__ sell_ptr,
+                                                     const int32_t* __restrict__ slice_rounds,
+                                                     int64_t rstride, KT* __restrict__ Ks,
+                                                     int64_t nslices, int64_t nn) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nslices) return;
+  const int64_t r = kSellRows * s + lane % kSellRows;
+  const int ph = lane / kSellRows;
+  const int32_t b0 = sell_ptr[s], rounds = slice_rounds[s];
+  int32_t rs = 0, len = 0;
+  if (r < nn) {
+    rs = row_ptr[r];
+    len = row_ptr[r + 1] - rs;
+  }
+  for (int m = 0; m < rounds; ++m) {
+    const int k = kSellLanes * m + ph;
+    KT* o = Ks + 9 * (b0 + rstride * m) + lane;
+    if (k < len) {
+      const KT* src = K + 9 * (int64_t)(rs + k);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) o[64 * v] = src[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) o[64 * v] = KT(0);
+    }
+  }
+}
+
+// pcg_A on the sliced-ELL tangent: one wave per slice of kSellRows block rows,
+// kSellLanes lanes (slot phases) per row combined by one shuffle stage.
+// Workgroup w owns rows [r0, r0 + chunk) (whole slices); its waves stride over
+// those slices.
+template <typename KT>
+__global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A(const int32_t* __restrict__ sell_ptr,
+                                                const int32_t* __restrict__ slice_rounds,
+                                                int64_t rstride,
+                                                const int32_t* __restrict__ scol,
+                                                const KT* __restrict__ sval, const KT* __restrict__ z,
+                                                   const KT* __restrict__ pold, KT* __restrict__ pnew,
+                                                   KT* __restrict__ q, const double* __restrict__ prz_cur,
+                                                   const double* __restrict__ prz_prev,
+                                                   const double* __restrict__ prr,
+                                                   double* __restrict__ ppq, PcgState* __restrict__ st,
+                                                   int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[3 * kSpmvThreads / 64];
   if (st->done | st->breakdown) return;
   const int k = st->k;
-  const double rz = sum_parts(prz_cur, np, sh);
-  const double rr = sum_parts(prr, np, sh);
-  double beta = 0.0;
-  if (k > 0) beta = rz / sum_parts(prz_prev, np, sh);
+  double rz, rr, rzp;
+  sum_parts3<kSpmvThreads>(prz_cur, prr, k > 0 ? prz_prev : nullptr, np, sh, rz, rr, rzp);
+  const double beta = k > 0 ? rz / rzp : 0.0;
   const bool conv = rr <= st->tol2;
   if (conv || k >= st->maxiter) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -223,33 +328,41 @@ __global__ __launch_bounds__(NT) void k_pcg_A(const int32_t* __restrict__ row_pt
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_last = rr;
   const KT bK = (KT)beta;
+  const bool first = (k == 0);
   int64_t r0, r1;
   wg_rows(np, chunk, nn, r0, r1);
-  const int g = threadIdx.x / kRowGroup, l = threadIdx.x % kRowGroup;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s_end = (r1 + kSellRows - 1) / kSellRows;
   double pq = 0;
-  for (int64_t i = r0 + g; i < r1; i += NT / kRowGroup) {
-    const int32_t rs = row_ptr[i], re = row_ptr[i + 1];
+  for (int64_t s = r0 / kSellRows + wave; s < s_end; s += kSpmvThreads / 64) {
+    const int32_t b0 = sell_ptr[s];
+    const int32_t rounds = slice_rounds[s];
+    const int64_t i = kSellRows * s + lane % kSellRows;
+    const int32_t* cp = scol + b0 + lane;
+    const KT* vp = sval + 9u * (uint32_t)b0 + lane;
+    const uint32_t rs = (uint32_t)rstride;
     KT a0 = 0, a1 = 0, a2 = 0;
-    for (int32_t kk = rs + l; kk < re; kk += kRowGroup) {
-      const int64_t j = col[kk];
-      const KT* v = val + 9 * (int64_t)kk;
-      KT p0 = z[3 * j], p1 = z[3 * j + 1], p2 = z[3 * j + 2];
-      if (k > 0) {
-        p0 += bK * pold[3 * j];
-        p1 += bK * pold[3 * j + 1];
-        p2 += bK * pold[3 * j + 2];
+#pragma unroll ONSAS_SPMV_UNROLL
+    for (int m = 0; m < rounds; ++m) {
+      const uint32_t j = 3u * (uint32_t)cp[rs * m];
+      const KT* v = vp + 9u * rs * m;
+      KT p0 = z[j], p1 = z[j + 1], p2 = z[j + 2];
+      if (!first) {
+        p0 += bK * pold[j];
+        p1 += bK * pold[j + 1];
+        p2 += bK * pold[j + 2];
       }
-      a0 += v[0] * p0 + v[1] * p1 + v[2] * p2;
-      a1 += v[3] * p0 + v[4] * p1 + v[5] * p2;
-      a2 += v[6] * p0 + v[7] * p1 + v[8] * p2;
+      a0 += v[0] * p0 + v[64] * p1 + v[128] * p2;
+      a1 += v[192] * p0 + v[256] * p1 + v[320] * p2;
+      a2 += v[384] * p0 + v[448] * p1 + v[512] * p2;
     }
 #pragma unroll
-    for (int o = kRowGroup / 2; o > 0; o >>= 1) {
-      a0 += __shfl_xor(a0, o, kRowGroup);
-      a1 += __shfl_xor(a1, o, kRowGroup);
-      a2 += __shfl_xor(a2, o, kRowGroup);
+    for (int o = kSellRows; o < 64; o <<= 1) {
+      a0 += __shfl_xor(a0, o, 64);
+      a1 += __shfl_xor(a1, o, 64);
+      a2 += __shfl_xor(a2, o, 64);
     }
-    if (l == 0) {
+    if (lane < kSellRows && i < r1) {
       KT p0 = z[3 * i], p1 = z[3 * i + 1], p2 = z[3 * i + 2];
       if (k > 0) {
         p0 += bK * pold[3 * i];
@@ -265,23 +378,23 @@ __global__ __launch_bounds__(NT) void k_pcg_A(const int32_t* __restrict__ row_pt
       pq += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
     }
   }
-  pq = block_sum(pq, sh);
+  pq = block_sum<kSpmvThreads>(pq, sh);
   if (threadIdx.x == 0) ppq[blockIdx.x] = pq;
 }
 
 template <typename KT>
-__global__ __launch_bounds__(NT) void k_pcg_B(const KT* __restrict__ p, const KT* __restrict__ q,
-                                                const KT* __restrict__ Dinv, KT* __restrict__ x,
-                                                KT* __restrict__ r, KT* __restrict__ z,
-                                                const double* __restrict__ prz_cur,
-                                                const double* __restrict__ ppq,
-                                                double* __restrict__ prz_next,
-                                                double* __restrict__ prr, PcgState* __restrict__ st,
-                                                int np, int64_t chunk, int64_t nn) {
-  __shared__ double sh[NT / 64];
+__global__ __launch_bounds__(NT, 8) void k_pcg_B(const KT* __restrict__ p, const KT* __restrict__ q,
+                                                   const KT* __restrict__ Dinv, KT* __restrict__ x,
+                                                   KT* __restrict__ r, KT* __restrict__ z,
+                                                   const double* __restrict__ prz_cur,
+                                                   const double* __restrict__ ppq,
+                                                   double* __restrict__ prz_next,
+                                                   double* __restrict__ prr, PcgState* __restrict__ st,
+                                                   int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
-  const double rz = sum_parts(prz_cur, np, sh);
-  const double pq = sum_parts(ppq, np, sh);
+  double rz, pq, unused;
+  sum_parts3(prz_cur, ppq, nullptr, np, sh, rz, pq, unused);
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
@@ -376,8 +489,10 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
   double* RZ[2] = {part(c, kPartRZ0), part(c, kPartRZ1)};
   PcgState* st = &c->d_sc->pcg;
   if (ea) (void)hipEventRecord(ea, s);
-  hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(NT), 0, s, c->d_row_ptr, c->d_col,
-                     (const KT*)c->d_K, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
+  const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
+  hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
+                     c->d_slice_rounds, rstride, c->d_sell_col,
+                     (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
                      RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, c->chunk,
                      c->nn);
   if (eb) (void)hipEventRecord(eb, s);
@@ -407,22 +522,30 @@ int build_pcg_graph(Ctx* c, int iters) {
 }
 
 template <typename KT>
-int pcg_solve_t(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
-                int* breakdown) {
+int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
+                int* converged, int* breakdown) {
   hipStream_t s = c->stream;
   const int np = c->np;
-  const double rtol = c->lin.rtol > 0 ? c->lin.rtol
-                                      : (c->precision == ONSAS_PRECISION_FP64 ? 1.4901161193847656e-08 : 1e-6);
+  const double rtol = linear_rtol(c);
   const int maxiter = c->lin.maxiter > 0 ? c->lin.maxiter : (int)std::min<int64_t>(c->nfree, 1 << 30);
   int chunk_iters = c->lin.check_every > 0 ? c->lin.check_every : 16;
   chunk_iters += chunk_iters & 1;  // even: the parity ping-pong repeats per chunk
   hipLaunchKernelGGL(k_pcg_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
                      (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz,
                      part(c, kPartRZ0), part(c, kPartRR), part(c, kPartBB), np, c->chunk, c->nn);
-  hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(NT), 0, s, part(c, kPartBB), &c->d_sc->pcg,
-                     rtol, maxiter, np);
+  const bool f = forcing && c->lin.forcing;
+  hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(NT), 0, s, part(c, kPartBB),
+                     part(c, kPartF2), &c->d_sc->pcg, &c->d_sc->nwt, rtol, maxiter, np, f ? 1 : 0,
+                     c->lin.newton_tol, c->lin.eta_max > 0 ? c->lin.eta_max : 0.1,
+                     c->lin.eta_first > 0 ? c->lin.eta_first : 1e-3);
   ONSAS_HIP_TRY(c, hipGetLastError());
-  ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
+  // ONSAS_PCG_GRAPH=0 replays the chunks as direct launches (same kernels, same
+  // order); profilers that intercept graph launches need it.
+  static const bool use_graph = [] {
+    const char* v = std::getenv("ONSAS_PCG_GRAPH");
+    return !(v && v[0] == '0');
+  }();
+  if (use_graph) ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
   // First chunk: direct launches with every pcg_A bracketed by events on this
   // stream (the sampled duration of the SpMV kernel); then graph replays.
   const int nsample = std::min<int>(chunk_iters, (int)(sizeof(c->ev_spmv) / sizeof(hipEvent_t)) / 2) & ~1;
@@ -432,7 +555,14 @@ int pcg_solve_t(Ctx* c, const double* b_dev, int* iters, double* relres, int* co
   ONSAS_HIP_TRY(c, hipGetLastError());
   bool first = true;
   for (;;) {
-    if (!first) ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph, s));
+    if (!first) {
+      if (use_graph) {
+        ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph, s));
+      } else {
+        for (int it = 0; it < chunk_iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
+        ONSAS_HIP_TRY(c, hipGetLastError());
+      }
+    }
     ONSAS_HIP_TRY(c, hipMemcpyAsync(&c->h_sc->pcg, &c->d_sc->pcg, sizeof(PcgState),
                                     hipMemcpyDeviceToHost, s));
     ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
@@ -477,11 +607,21 @@ int launch_residual(Ctx* c) {
   return ONSAS_OK;
 }
 
-int launch_pcg_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged,
-                     int* breakdown) {
+int launch_pcg_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
+                     int* converged, int* breakdown) {
   return c->precision == ONSAS_PRECISION_FP64
-             ? pcg_solve_t<double>(c, b_dev, iters, relres, converged, breakdown)
-             : pcg_solve_t<float>(c, b_dev, iters, relres, converged, breakdown);
+             ? pcg_solve_t<double>(c, b_dev, forcing, iters, relres, converged, breakdown)
+             : pcg_solve_t<float>(c, b_dev, forcing, iters, relres, converged, breakdown);
+}
+
+double linear_rtol(const Ctx* c) {
+  return c->lin.rtol > 0 ? c->lin.rtol
+                         : (c->precision == ONSAS_PRECISION_FP64 ? 1.4901161193847656e-08 : 1e-6);
+}
+
+int reset_forcing(Ctx* c) {
+  ONSAS_HIP_TRY(c, hipMemsetAsync(&c->d_sc->nwt.prev_b2, 0, sizeof(double), c->stream));
+  return ONSAS_OK;
 }
 
 int launch_update(Ctx* c) {
@@ -502,6 +642,21 @@ int launch_matvec(Ctx* c, const double* x_dev, double* y_dev) {
   else
     hipLaunchKernelGGL(k_spmv_f64<float>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
                        c->d_col, (const float*)c->d_K, x_dev, y_dev, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_pack_sell(Ctx* c) {
+  const unsigned grid = grid_for(c->nslices, 4);
+  const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_pack_sell<double>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       (const double*)c->d_K, c->d_sell_ptr, c->d_slice_rounds, rstride,
+                       (double*)c->d_Ksell, c->nslices, c->nn);
+  else
+    hipLaunchKernelGGL(k_pack_sell<float>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       (const float*)c->d_K, c->d_sell_ptr, c->d_slice_rounds, rstride,
+                       (float*)c->d_Ksell, c->nslices, c->nn);
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
 }

@@ -10,7 +10,8 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libonsas_hip.so")
+LIB_PATH = os.environ.get("ONSAS_HIP_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libonsas_hip.so")
 
 # status codes (include/onsas_hip.h)
 OK = 0
@@ -38,7 +39,9 @@ EXPORTED = (
 
 class LinearSettings(C.Structure):
     _fields_ = [("kind", C.c_int32), ("maxiter", C.c_int32), ("rtol", C.c_double),
-                ("gmres_restart", C.c_int32), ("check_every", C.c_int32)]
+                ("gmres_restart", C.c_int32), ("check_every", C.c_int32),
+                ("forcing", C.c_int32), ("newton_tol", C.c_double), ("eta_max", C.c_double),
+                ("eta_first", C.c_double)]
 
 
 class StepStats(C.Structure):
@@ -46,7 +49,7 @@ class StepStats(C.Structure):
                 ("r_rel", C.c_double), ("fext_norm", C.c_double), ("U_norm", C.c_double),
                 ("lin_rel_residual", C.c_double), ("lin_iters", C.c_int32),
                 ("lin_solver", C.c_int32), ("lin_converged", C.c_int32), ("reserved", C.c_int32),
-                ("assemble_ms", C.c_double), ("solve_ms", C.c_double)]
+                ("assemble_ms", C.c_double), ("solve_ms", C.c_double), ("lin_rtol", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -217,9 +220,11 @@ class Context:
         self._check(self.L.onsas_finalize(self.h))
 
     def set_linear_solver(self, kind=LINSOLVE_AUTO, rtol=0.0, maxiter=0, gmres_restart=40,
-                          check_every=16):
+                          check_every=16, forcing=False, newton_tol=0.0, eta_max=0.0,
+                          eta_first=0.0):
         s = LinearSettings(int(kind), int(maxiter), float(rtol), int(gmres_restart),
-                           int(check_every))
+                           int(check_every), int(bool(forcing)), float(newton_tol),
+                           float(eta_max), float(eta_first))
         self._check(self.L.onsas_set_linear_solver(self.h, C.byref(s)))
 
     # -- per step / iteration

@@ -293,6 +293,8 @@ def _solve_nonlinear_(sa, alg, backend):
     s = sa.s
     if sa.state is None:
         sa.state = GpuStaticState(s, backend)
+    if backend.forcing:
+        sa.state.configure_linear_solver(newton_tol=tolerances(alg).rel_res_force_tol)
     if sa.initial_state is not None:
         sa.state.ctx.set_U(sa.initial_state)
     sol = Solution(sa, alg)

@@ -34,6 +34,11 @@ class MI355XBackend:
     gmres_restart: int = 40
     check_every: int = 16
     store_stresses: bool = True
+    # inexact Newton: per-iteration inner tolerance (Eisenstat-Walker with a
+    # floor at what the outer tolerance needs); rtol is then the tightest one
+    forcing: bool = False
+    eta_max: float = 0.1
+    eta_first: float = 1e-2
 
     def precision_code(self):
         if self.precision not in ("mixed", "fp64"):
@@ -82,10 +87,15 @@ class GpuStaticState:
             self.pressure_bcs.append(bc)
         ctx.set_fixed_dofs(structure.fixed_dofs)
         ctx.finalize()
-        ctx.set_linear_solver(_SOLVERS[backend.solver], backend.rtol or 0.0, backend.maxiter or 0,
-                              backend.gmres_restart, backend.check_every)
+        self.configure_linear_solver()
         self.n = m.num_dofs()
         self.free_dofs = structure.free_dofs
+
+    def configure_linear_solver(self, newton_tol=0.0):
+        b = self.backend
+        self.ctx.set_linear_solver(_SOLVERS[b.solver], b.rtol or 0.0, b.maxiter or 0,
+                                   b.gmres_restart, b.check_every, b.forcing, newton_tol,
+                                   b.eta_max, b.eta_first)
 
     # -- loads
     def apply_loads(self, t):

@@ -239,6 +239,23 @@ def test_nh_compression_vs_oracle(oracle, prec):
         np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-8 * scale)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_forcing_converges_to_same_state(prec):
+    """Inexact-Newton forcing changes only the inner tolerances: same residual
+    criterion, same converged displacements, fewer inner iterations."""
+    print(f"[gpu] forcing vs fixed {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=8, jitter=0.05, seed=2)
+    fixed = _newton(s, 5, prec)
+    forced = _newton(s, 5, prec, forcing=True)
+    for k in range(5):
+        scale = np.abs(fixed.U[k]).max()
+        np.testing.assert_allclose(forced.U[k], fixed.U[k], rtol=0, atol=1e-8 * scale)
+        assert forced.records[k]["dr_rel"] <= 1e-10
+    assert forced.linear_iterations() < fixed.linear_iterations()
+    etas = [d["lin_rtol"] for d in forced.newton_log]
+    assert max(etas) <= 0.1 and min(etas) >= (1e-6 if prec == "mixed" else 1.4e-8)
+
+
 def test_trusses_green_clamped(golden):
     """Clamped Green-strain bar (examples/clamped_truss) through the API on the GPU."""
     print("[gpu] clamped truss", flush=True)
