@@ -2,7 +2,7 @@
 // block-Jacobi preconditioned CG on the 3x3 block-CSR tangent.  gfx950, wave64.
 //
 // Reductions are deterministic and atomic-free: every reduction-producing
-// kernel runs a fixed grid of np workgroups x 1024 threads; each workgroup
+// kernel runs a fixed grid of np workgroups x 512 threads; each workgroup
 // reduces with a fixed shuffle tree + fixed-order LDS sum and writes one
 // partial; the consumer kernel (every workgroup of the next launch) re-sums the
 // np partials in index order.  Kernel boundaries give the visibility, so no
@@ -18,8 +18,10 @@
 //          partials r.z and r.r;  breakdown test p.q <= 0 (indefinite tangent).
 // Chunks of iterations are captured once in a HIP graph and replayed; the host
 // reads the device state between chunks.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -480,6 +482,138 @@ __global__ void k_convert(const KT* __restrict__ x, double* __restrict__ out, in
 
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// ---- GMRES building blocks (fallback when the tangent is indefinite) ----------
+
+// y = K x on the sliced-ELL tangent (one wave per slice, no reductions)
+template <typename KT>
+__global__ __launch_bounds__(256) void k_spmv_sell(const int32_t* __restrict__ sell_ptr,
+                                                     const int32_t* __restrict__ slice_rounds,
+                                                     int64_t rstride, const int32_t* __restrict__ scol,
+                                                     const KT* __restrict__ sval,
+                                                     const KT* __restrict__ x, KT* __restrict__ y,
+                                                     int64_t nslices, int64_t nn) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nslices) return;
+  const int32_t b0 = sell_ptr[s];
+  const int32_t rounds = slice_rounds[s];
+  const uint32_t rs = (uint32_t)rstride;
+  const int32_t* cp = scol + b0 + lane;
+  const KT* vp = sval + 9u * (uint32_t)b0 + lane;
+  KT a0 = 0, a1 = 0, a2 = 0;
+  for (int m = 0; m < rounds; ++m) {
+    const uint32_t j = 3u * (uint32_t)cp[rs * m];
+    const KT* v = vp + 9u * rs * m;
+    const KT p0 = x[j], p1 = x[j + 1], p2 = x[j + 2];
+    a0 += v[0] * p0 + v[64] * p1 + v[128] * p2;
+    a1 += v[192] * p0 + v[256] * p1 + v[320] * p2;
+    a2 += v[384] * p0 + v[448] * p1 + v[512] * p2;
+  }
+#pragma unroll
+  for (int o = kSellRows; o < 64; o <<= 1) {
+    a0 += __shfl_xor(a0, o, 64);
+    a1 += __shfl_xor(a1, o, 64);
+    a2 += __shfl_xor(a2, o, 64);
+  }
+  const int64_t i = kSellRows * s + lane % kSellRows;
+  if (lane < kSellRows && i < nn) {
+    y[3 * i] = a0;
+    y[3 * i + 1] = a1;
+    y[3 * i + 2] = a2;
+  }
+}
+
+template <typename T>
+__global__ void k_fill_kt(T* __restrict__ p, int64_t n, T v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// t = D^-1 v (block-Jacobi, per node)
+template <typename KT>
+__global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, KT* __restrict__ t,
+                         int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  KT r[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]}, z[3];
+  jacobi_apply(Dinv + 9 * i, r, z);
+  t[3 * i] = z[0];
+  t[3 * i + 1] = z[1];
+  t[3 * i + 2] = z[2];
+}
+
+// partial[i * np + wg] = sum over the workgroup's rows of V_i . w   (i < nvec)
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_multidot(const KT* __restrict__ V, int64_t ldv, int nvec,
+                                                   const KT* __restrict__ w, double* __restrict__ part,
+                                                   int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  for (int i = 0; i < nvec; ++i) {
+    const KT* v = V + ldv * i;
+    double s = 0;
+    for (int64_t k = 3 * r0 + threadIdx.x; k < 3 * r1; k += NT) s += (double)v[k] * (double)w[k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[(int64_t)i * np + blockIdx.x] = s;
+  }
+}
+
+// out[i] = sum_wg partial[i * np + wg] in index order (one workgroup)
+__global__ __launch_bounds__(NT) void k_reduce_rows(const double* __restrict__ part, int nvec, int np,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[NT / 64];
+  for (int i = 0; i < nvec; ++i) {
+    const double s = sum_parts(part + (int64_t)i * np, np, sh);
+    if (threadIdx.x == 0) out[i] = s;
+  }
+}
+
+// w -= sum_i h_i V_i  (fixed order over i)
+template <typename KT>
+__global__ void k_sub_combination(KT* __restrict__ w, const KT* __restrict__ V, int64_t ldv,
+                                  const double* __restrict__ h, int nvec, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0;
+  for (int i = 0; i < nvec; ++i) s += h[i] * (double)V[ldv * i + k];
+  w[k] = (KT)((double)w[k] - s);
+}
+
+template <typename KT>
+__global__ void k_scale_into(const KT* w, KT* v, double f, int64_t n) {  // may run in place
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) v[k] = (KT)((double)w[k] * f);
+}
+
+// x += D^-1 (sum_i y_i V_i), per node
+template <typename KT>
+__global__ void k_update_from_basis(KT* __restrict__ x, const KT* __restrict__ V, int64_t ldv,
+                                    const double* __restrict__ y, int nvec,
+                                    const KT* __restrict__ Dinv, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  KT t[3], z[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double s = 0;
+    for (int q = 0; q < nvec; ++q) s += y[q] * (double)V[ldv * q + 3 * i + d];
+    t[d] = (KT)s;
+  }
+  jacobi_apply(Dinv + 9 * i, t, z);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[3 * i + d] += z[d];
+}
+
+// v = b - K x on free dofs (0 on fixed)
+template <typename KT>
+__global__ void k_residual_kt(const double* __restrict__ b, const KT* __restrict__ Kx,
+                              const uint8_t* __restrict__ mask, KT* __restrict__ v, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  v[k] = ((mask[k / 3] >> (k % 3)) & 1u) ? KT(0) : (KT)(b[k] - (double)Kx[k]);
+}
+
 // One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
 // optional events around pcg_A.
 template <typename KT>
@@ -685,11 +819,128 @@ int launch_convert_K(Ctx* c, double* out_dev) {
   return ONSAS_OK;
 }
 
-int launch_gmres_solve(Ctx* c, const double*, int* iters, double* relres, int* converged) {
-  *iters = 0;
-  *relres = 1.0;
-  *converged = 0;
-  return set_err(c, ONSAS_ERR_UNSUPPORTED, "GMRES fallback not available");
+namespace {
+
+// Restarted GMRES(m), right block-Jacobi preconditioning (K D^-1 y = b,
+// x = D^-1 y), classical Gram-Schmidt applied twice, Givens rotations on the
+// host in FP64.  The basis lives in HBM in the tangent precision; every dot
+// product is a deterministic two-level reduction.  Result in the PCG x buffer.
+template <typename KT>
+int gmres_solve_t(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged) {
+  hipStream_t s = c->stream;
+  const int m = std::max(2, c->lin.gmres_restart > 0 ? c->lin.gmres_restart : 40);
+  const int64_t n = 3 * c->nn;
+  if (c->gm_m < m) {
+    for (void* p : {c->d_gm_V, c->d_gm_w, (void*)c->d_gm_h})
+      if (p) (void)hipFree(p);
+    c->d_gm_V = c->d_gm_w = nullptr;
+    c->d_gm_h = nullptr;
+    c->gm_m = 0;
+    ONSAS_HIP_TRY(c, hipMalloc(&c->d_gm_V, sizeof(KT) * (size_t)(m + 1) * n));
+    ONSAS_HIP_TRY(c, hipMalloc(&c->d_gm_w, sizeof(KT) * (size_t)n));
+    ONSAS_HIP_TRY(c, hipMalloc((void**)&c->d_gm_h,
+                               sizeof(double) * ((size_t)(m + 2) * kMaxPartials + 2 * (m + 2))));
+    c->gm_m = m;
+  }
+  KT* V = (KT*)c->d_gm_V;
+  KT* w = (KT*)c->d_gm_w;
+  KT* t = (KT*)c->d_pq;
+  KT* x = (KT*)c->d_px;
+  const KT* Dinv = (const KT*)c->d_Dinv;
+  double* dpart = c->d_gm_h;
+  double* dres = dpart + (size_t)(m + 2) * kMaxPartials;
+  double* dcoef = dres + (m + 2);
+  const double rtol = linear_rtol(c);
+  const int maxiter = c->lin.maxiter > 0 ? c->lin.maxiter : (int)std::min<int64_t>(c->nfree, 1 << 30);
+  const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
+  const unsigned gn = grid_for(n, 256), gnn = grid_for(c->nn, 256), gs = grid_for(c->nslices, 4);
+
+  auto spmv = [&](const KT* in, KT* out) {
+    hipLaunchKernelGGL(k_spmv_sell<KT>, dim3(gs), dim3(256), 0, s, c->d_sell_ptr, c->d_slice_rounds,
+                       rstride, c->d_sell_col, (const KT*)c->d_Ksell, in, out, c->nslices, c->nn);
+  };
+  std::vector<double> hbuf(m + 2);
+  auto dots = [&](const KT* Vb, int nvec, const KT* vec, double* out) -> int {
+    hipLaunchKernelGGL(k_multidot<KT>, dim3(c->np), dim3(NT), 0, s, Vb, n, nvec, vec, dpart, c->np,
+                       c->chunk, c->nn);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(NT), 0, s, dpart, nvec, c->np, dres);
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(out, dres, sizeof(double) * nvec, hipMemcpyDeviceToHost, s));
+    ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
+    return ONSAS_OK;
+  };
+
+  hipLaunchKernelGGL(k_fill_kt<KT>, dim3(gn), dim3(256), 0, s, x, n, KT(0));
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  double beta0 = -1, beta = 0;
+  int total = 0;
+  for (;;) {
+    spmv(x, t);
+    hipLaunchKernelGGL(k_residual_kt<KT>, dim3(gn), dim3(256), 0, s, b_dev, t, c->d_fixmask, V, n);
+    double bb;
+    ONSAS_TRY(dots(V, 1, V, &bb));
+    beta = std::sqrt(bb);
+    if (beta0 < 0) beta0 = beta;
+    if (!(beta > rtol * beta0) || total >= maxiter || beta0 == 0) break;
+    hipLaunchKernelGGL(k_scale_into<KT>, dim3(gn), dim3(256), 0, s, V, V, 1.0 / beta, n);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+      hipLaunchKernelGGL(k_jacobi<KT>, dim3(gnn), dim3(256), 0, s, Dinv, V + n * j, t, c->nn);
+      spmv(t, w);
+      for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
+        ONSAS_TRY(dots(V, j + 1, w, hbuf.data()));
+        ONSAS_HIP_TRY(c, hipMemcpyAsync(dcoef, hbuf.data(), sizeof(double) * (j + 1),
+                                        hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_sub_combination<KT>, dim3(gn), dim3(256), 0, s, w, V, n, dcoef, j + 1, n);
+        for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] += hbuf[i];
+      }
+      double ww;
+      ONSAS_TRY(dots(w, 1, w, &ww));
+      const double hn = std::sqrt(ww);
+      H[(size_t)(j + 1) * m + j] = hn;
+      if (hn > 0)
+        hipLaunchKernelGGL(k_scale_into<KT>, dim3(gn), dim3(256), 0, s, w, V + n * (j + 1), 1.0 / hn, n);
+      for (int i = 0; i < j; ++i) {
+        const double a = H[(size_t)i * m + j], bq = H[(size_t)(i + 1) * m + j];
+        H[(size_t)i * m + j] = cs[i] * a + sn[i] * bq;
+        H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bq;
+      }
+      const double a = H[(size_t)j * m + j], bq = H[(size_t)(j + 1) * m + j];
+      const double r = std::hypot(a, bq);
+      cs[j] = r > 0 ? a / r : 1.0;
+      sn[j] = r > 0 ? bq / r : 0.0;
+      H[(size_t)j * m + j] = r;
+      H[(size_t)(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++total;
+      k = j + 1;
+      if (std::fabs(g[j + 1]) <= rtol * beta0 || total >= maxiter || !(hn > 0)) break;
+    }
+    for (int i = k - 1; i >= 0; --i) {  // H y = g
+      double sacc = g[i];
+      for (int l = i + 1; l < k; ++l) sacc -= H[(size_t)i * m + l] * y[l];
+      y[i] = H[(size_t)i * m + i] != 0 ? sacc / H[(size_t)i * m + i] : 0.0;
+    }
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(dcoef, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_update_from_basis<KT>, dim3(gnn), dim3(256), 0, s, x, V, n, dcoef, k, Dinv,
+                       c->nn);
+    ONSAS_HIP_TRY(c, hipGetLastError());
+  }
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  *iters = total;
+  *relres = beta0 > 0 ? beta / beta0 : 0.0;
+  *converged = (beta0 == 0 || beta <= rtol * beta0) ? 1 : 0;
+  return ONSAS_OK;
+}
+
+}  // namespace
+
+int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged) {
+  return c->precision == ONSAS_PRECISION_FP64 ? gmres_solve_t<double>(c, b_dev, iters, relres, converged)
+                                              : gmres_solve_t<float>(c, b_dev, iters, relres, converged);
 }
 
 }  // namespace onsas

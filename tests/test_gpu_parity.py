@@ -252,8 +252,74 @@ def test_forcing_converges_to_same_state(prec):
         np.testing.assert_allclose(forced.U[k], fixed.U[k], rtol=0, atol=1e-8 * scale)
         assert forced.records[k]["dr_rel"] <= 1e-10
     assert forced.linear_iterations() < fixed.linear_iterations()
-    etas = [d["lin_rtol"] for d in forced.newton_log]
-    assert max(etas) <= 0.1 and min(etas) >= (1e-6 if prec == "mixed" else 1.4e-8)
+    floor = 1e-6 if prec == "mixed" else 1.4e-8
+    for d in forced.newton_log:
+        # eta_max = 0.1 caps the tolerance; only the solve whose residual already
+        # meets the outer tolerance (r_rel <= 1e-10, kept by isconverged's
+        # residual-first order) may be loose up to 0.5
+        cap = 0.5 if d["r_rel"] <= 1e-10 else 0.1
+        assert floor * (1 - 1e-12) <= d["lin_rtol"] <= cap, d
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gmres_matches_direct(oracle, prec):
+    """Restarted GMRES (block-Jacobi right preconditioning) on the assembled
+    tangent vs a sparse direct solve, with restarts exercised (m = 8)."""
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] gmres vs direct {prec}", flush=True)
+    s, sm = workloads.cube_extension(n=4, jitter=0.1, seed=4)
+    fixed = s.fixed_dofs
+    c = _ctx(prec, sm.coords, sm.tets, "svk", fixed=fixed)
+    c.set_linear_solver(_capi.LINSOLVE_GMRES, rtol=1e-10 if prec == "fp64" else 1e-6,
+                        maxiter=3000, gmres_restart=8)
+    c.assemble()
+    b = np.random.default_rng(7).normal(size=3 * sm.nn)
+    x, it, rr = c.linear_solve(b)
+    Kff = _oracle_model(oracle, sm.coords, sm.tets, "svk", fixed).assemble(
+        np.zeros(3 * sm.nn), want_full=False, want_ff=True)["Kff"]
+    free = np.setdiff1d(np.arange(3 * sm.nn), fixed)
+    xo = spla.spsolve(Kff.tocsc(), b[free])
+    err = np.abs(x[free] - xo).max() / np.abs(xo).max()
+    assert err < (1e-8 if prec == "fp64" else 1e-3), err
+    assert it > 8 and np.all(x[fixed] == 0)
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gmres_fallback_on_indefinite_tangent(prec):
+    """AUTO: PCG breaks down on an indefinite tangent (a compressed bar chain
+    has negative transverse stiffness) and GMRES solves the system."""
+    print(f"[gpu] gmres fallback {prec}", flush=True)
+    X = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0.0], [3, 0.1, 0.2]])
+    c = _ctx(prec, X, trusses=[[0, 1], [1, 2], [2, 3]], E=1.0, A=[1.0, 1.0, 1.0],
+             fixed=[0, 1, 2, 9, 10, 11])
+    c.set_linear_solver(_capi.LINSOLVE_AUTO, rtol=1e-10 if prec == "fp64" else 1e-6, maxiter=200)
+    U = np.zeros(12)
+    U[6] = -0.2
+    U[9] = -0.3
+    c.set_U(U)
+    c.assemble()
+    b = np.array([0, 0, 0, 0.1, 1.0, -0.3, 0.2, 0.5, 0.4, 0, 0, 0.0])
+    x, it, rr = c.linear_solve(b)
+    K = c.tangent_bsr().toarray()
+    free = [3, 4, 5, 6, 7, 8]
+    assert np.linalg.eigvalsh(K[np.ix_(free, free)]).min() < 0  # genuinely indefinite
+    res = K[np.ix_(free, free)] @ x[free] - b[free]
+    assert np.linalg.norm(res) <= (1e-9 if prec == "fp64" else 1e-5) * np.linalg.norm(b[free])
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_newton_with_gmres_only(prec):
+    print(f"[gpu] newton with gmres {prec}", flush=True)
+    s, sm = workloads.cube_extension(n=3)
+    sol = _newton(s, 3, prec, solver="gmres", gmres_restart=20)
+    for k, t in enumerate(np.linspace(1 / 3, 1.0, 3)):
+        a = workloads.svk_uniaxial_alpha(1e6, 3e5 * t)
+        b = workloads.svk_uniaxial_beta(a, 0.3)
+        ex = workloads.homogeneous_displacement(sm.coords, a, b)
+        np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-8 * np.abs(ex).max())
+    assert all(d["lin_solver"] == 2 for d in sol.newton_log)
 
 
 def test_trusses_green_clamped(golden):
