@@ -221,12 +221,22 @@ int onsas_linear_solve(onsas_ctx* ctx, const double* b, double* x, int32_t* iter
                        double* rel_residual);
 
 /* Adjoint sensitivity of J(U) to the per-element shear moduli of a tet group
- * (no counterpart in the reference; the rrule of the solve): solves K lambda =
- * dJ/dU on free dofs with the assembled (symmetric) tangent, then
- * dJdG[e] = -lambda_e . d f_int,e / d G_e.  dJdU: n values; dJdG: ne values in
- * user order.  Requires a preceding onsas_assemble at the converged state. */
+ * (no counterpart in the reference; the pullback of the solve, for a
+ * ChainRulesCore rrule of solve! on the NeoHookean / SVK shear modulus of
+ * src/Materials/NeoHookeanMaterial.jl:94-108 and SVKMaterial.jl:89-100).
+ * Assembles K at the current U (the converged state), solves K lambda = dJ/dU
+ * on the free dofs to ||dJ/dU - K lambda|| <= 1e-11 ||dJ/dU|| by iterative
+ * refinement (FP64 element-wise residual, corrections with the configured
+ * solver: FP32 PCG in mixed mode), then dJdG[e] = -lambda_e . d f_int,e / d G_e.
+ * dJdU: n values (fixed dofs ignored); dJdG: ne values in user order; iters:
+ * total inner iterations. */
 int onsas_adjoint_shear(onsas_ctx* ctx, int32_t group, const double* dJdU, double* dJdG,
                         int32_t* iters);
+/* Replace the per-element shear moduli G of an SVK / NeoHookean tet group (ne
+ * values, user order) without rebuilding the model: the parameter update of a
+ * gradient-based inverse problem (no counterpart in the reference).  The next
+ * onsas_assemble uses the new moduli. */
+int onsas_set_shear_moduli(onsas_ctx* ctx, int32_t group, const double* G);
 
 /* Timers. */
 int onsas_get_timings(onsas_ctx* ctx, onsas_timings* t);

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -102,9 +103,84 @@ int linear_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* 
   return ONSAS_OK;
 }
 
+// Sums of the R2 / F2 partial rows written by launch_refine_residual (host,
+// index order, so the result is deterministic).
+int read_r2_b2(Ctx* c, double* r2, double* b2) {
+  std::vector<double> h(2 * (size_t)c->np);
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(h.data(), onsas::part(c, onsas::kPartR2), c->np * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(h.data() + c->np, onsas::part(c, onsas::kPartF2),
+                                  c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  double a = 0, b = 0;
+  for (int i = 0; i < c->np; ++i) {
+    a += h[i];
+    b += h[c->np + i];
+  }
+  *r2 = a;
+  *b2 = b;
+  return ONSAS_OK;
+}
+
+// Solve K(U) x = b on the free dofs to ||b - K x|| <= tol ||b|| by iterative
+// refinement: the residual uses the FP64 element-wise tangent product (the
+// blocks are re-evaluated in FP64, never stored), each correction is a solve
+// with the assembled tangent (FP32 in mixed mode) by the configured solver.
+// x, y, r: n doubles of device scratch.
+int refined_solve(Ctx* c, const double* b, double* x, double* y, double* r, double tol,
+                  int max_pass, int* iters, double* relres) {
+  const size_t nb = 3 * (size_t)c->nn * sizeof(double);
+  ONSAS_HIP_TRY(c, hipMemsetAsync(x, 0, nb, c->stream));
+  ONSAS_HIP_TRY(c, hipMemsetAsync(y, 0, nb, c->stream));
+  ONSAS_TRY(onsas::launch_refine_residual(c, b, y, r));
+  double r2 = 0, b2 = 0;
+  ONSAS_TRY(read_r2_b2(c, &r2, &b2));
+  *iters = 0;
+  *relres = 0;
+  if (!(b2 > 0)) return ONSAS_OK;
+  const double bn = std::sqrt(b2);
+  double rel = std::sqrt(r2) / bn;
+  for (int p = 0; p < max_pass && rel > tol; ++p) {
+    int it = 0, conv = 0, solver = 0;
+    double rr = 0;
+    ONSAS_TRY(linear_solve(c, r, false, &it, &rr, &conv, &solver));
+    *iters += it;
+    ONSAS_TRY(onsas::launch_accumulate_x(c, x));
+    ONSAS_TRY(onsas::launch_tangent_apply_f64(c, x, y));
+    ONSAS_TRY(onsas::launch_refine_residual(c, b, y, r));
+    ONSAS_TRY(read_r2_b2(c, &r2, &b2));
+    const double next = std::sqrt(r2) / bn;
+    if (!std::isfinite(next)) return set_err(c, ONSAS_ERR_NOT_FINITE, "non-finite refinement residual");
+    if (next > 0.5 * rel) {  // the correction no longer contracts
+      rel = next;
+      break;
+    }
+    rel = next;
+  }
+  *relres = rel;
+  if (rel > tol)
+    return set_err(c, ONSAS_ERR_LINEAR_SOLVE,
+                   "refined solve stalled at relative residual " + std::to_string(rel));
+  return ONSAS_OK;
+}
+
+// f_int and K at the current U (the body of onsas_assemble).
+int assemble_now(Ctx* c) {
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_Uasm, c->d_U, 3 * c->nn * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
+  ONSAS_TRY(onsas::launch_assemble(c));
+  ONSAS_TRY(onsas::launch_pack_sell(c));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  return ONSAS_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+// target relative residual of the adjoint solve (FP64 refinement)
+static constexpr double kAdjointTol = 1e-11;
 
 const char* onsas_version(void) { return "onsas_hip 0.1.0 (gfx950)"; }
 
@@ -299,12 +375,7 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
 int onsas_assemble(onsas_ctx* ctx, double* fint_out) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
-  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_Uasm, c->d_U, 3 * c->nn * sizeof(double),
-                                  hipMemcpyDeviceToDevice, c->stream));
-  ONSAS_TRY(onsas::launch_assemble(c));
-  ONSAS_TRY(onsas::launch_pack_sell(c));
-  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  ONSAS_TRY(assemble_now(c));
   if (fint_out)
     ONSAS_HIP_TRY(c, hipMemcpyAsync(fint_out, c->d_fint, 3 * c->nn * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream));
@@ -512,19 +583,47 @@ int onsas_adjoint_shear(onsas_ctx* ctx, int32_t group, const double* dJdU, doubl
   if (g.type != onsas::kTet || (g.mat != ONSAS_MAT_NEOHOOKEAN && g.mat != ONSAS_MAT_SVK))
     return set_err(c, ONSAS_ERR_UNSUPPORTED, "adjoint shear sensitivity needs an SVK or NeoHookean tet group");
   if (!dJdU || !dJdG) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
-  if (!c->assembled && c->tm.n_assemble == 0) return set_err(c, ONSAS_ERR_STATE, "assemble first");
   const size_t n = 3 * (size_t)c->nn;
-  DevScratch db, dout;
+  // the tangent at the current U (after a converged Newton loop U has moved by
+  // the last correction since the last assembly)
+  ONSAS_TRY(onsas_assemble(ctx, nullptr));
+  DevScratch db, dlam, dy, dr, dout;
   ONSAS_TRY(to_device(c, db, dJdU, n));
+  ONSAS_TRY(to_device(c, dlam, nullptr, n));
+  ONSAS_TRY(to_device(c, dy, nullptr, n));
+  ONSAS_TRY(to_device(c, dr, nullptr, n));
   ONSAS_TRY(to_device(c, dout, nullptr, g.ne));
-  int it = 0, conv = 0, solver = 0;
+  int it = 0;
   double rr = 0;
-  ONSAS_TRY(linear_solve(c, db.p, false, &it, &rr, &conv, &solver));
-  if (!conv) return set_err(c, ONSAS_ERR_LINEAR_SOLVE, "adjoint solve did not reach its tolerance");
-  ONSAS_TRY(onsas::launch_adjoint_shear(c, group, c->d_px, dout.p));
+  ONSAS_TRY(refined_solve(c, db.p, dlam.p, dy.p, dr.p, kAdjointTol, 8, &it, &rr));
+  ONSAS_TRY(onsas::launch_adjoint_shear(c, group, dlam.p, dout.p));
   ONSAS_HIP_TRY(c, hipMemcpyAsync(dJdG, dout.p, g.ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (iters) *iters = it;
+  return ONSAS_OK;
+}
+
+int onsas_set_shear_moduli(onsas_ctx* ctx, int32_t group, const double* G) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (group < 0 || group >= (int32_t)c->groups.size()) return set_err(c, ONSAS_ERR_ARG, "bad group");
+  ElemGroup& g = c->groups[group];
+  if (g.type != onsas::kTet || (g.mat != ONSAS_MAT_NEOHOOKEAN && g.mat != ONSAS_MAT_SVK))
+    return set_err(c, ONSAS_ERR_UNSUPPORTED, "per-element shear modulus needs an SVK or NeoHookean tet group");
+  if (!G) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
+  std::vector<double> Gs(g.ne);
+  for (int64_t s = 0; s < g.ne; ++s) {
+    const double v = G[g.perm[s]];
+    if (!std::isfinite(v)) return set_err(c, ONSAS_ERR_ARG, "non-finite shear modulus");
+    Gs[s] = v;
+  }
+  if (!g.d_G) ONSAS_HIP_TRY(c, hipMalloc((void**)&g.d_G, g.ne * sizeof(double)));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(g.d_G, Gs.data(), g.ne * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  g.per_elem_G = true;
+  g.Gel.assign(G, G + g.ne);
+  c->assembled = false;  // K and f_int belong to the old moduli
   return ONSAS_OK;
 }
 

@@ -354,6 +354,89 @@ __global__ __launch_bounds__(kScatterThreads) void k_truss_scatter(TrussArgs ta,
   }
 }
 
+// y += K(U) v with the element blocks evaluated in FP64 and never stored: the
+// FP64 tangent product used to refine mixed-precision solves.  Lanes, colours
+// and the fixed-dof masking are those of k_tet_scatter, so the operator is the
+// assembled one (fixed rows and columns are completed by k_fixed_copy).
+__global__ __launch_bounds__(kScatterThreads) void k_tet_apply(TetArgs ta,
+                                                               const double* __restrict__ U,
+                                                               const double* __restrict__ v,
+                                                               double* __restrict__ y,
+                                                               const uint8_t* __restrict__ mask) {
+  const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
+  const int64_t e = ta.e0 + (t >> 2);
+  const int a = (int)(t & 3);
+  if (e >= ta.e1) return;
+  int32_t nd[4];
+  double n[4][3], u[4][3], V, p1;
+  tet_load(ta, e, U, nd, n, V, u, p1);
+  TetEval ev;
+  tet_eval(ta.mat, ta.p0, p1, n, u, ev);
+  TetBlocks<double> tb;
+  tb.load(ev, n, V);
+  double acc[3] = {0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const uint32_t mb = mask[nd[b]];
+    double vb[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) vb[j] = ((mb >> j) & 1u) ? 0.0 : v[3 * (int64_t)nd[b] + j];
+    double blk[9];
+    tb.block(a, b, blk);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc[i] += blk[i * 3] * vb[0] + blk[i * 3 + 1] * vb[1] + blk[i * 3 + 2] * vb[2];
+  }
+  const uint32_t ma = mask[nd[a]];
+  double* yp = y + 3 * (int64_t)nd[a];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) yp[i] += ((ma >> i) & 1u) ? 0.0 : acc[i];
+}
+
+__global__ __launch_bounds__(kScatterThreads) void k_truss_apply(TrussArgs ta,
+                                                                 const double* __restrict__ U,
+                                                                 const double* __restrict__ v,
+                                                                 double* __restrict__ y,
+                                                                 const uint8_t* __restrict__ mask) {
+  const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
+  const int64_t e = ta.e0 + (t >> 1);
+  const int a = (int)(t & 1);
+  if (e >= ta.e1) return;
+  const int64_t ne = ta.ne;
+  const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
+  double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
+  const double l0 = ta.geo[3 * ne + e], A = ta.geo[4 * ne + e];
+  double du[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) du[d] = U[3 * (int64_t)nd1 + d] - U[3 * (int64_t)nd0 + d];
+  TrussEval tv;
+  truss_eval(ta.strain, ta.E, A, d0, l0, du, tv);
+  // K_ab v_b summed over b: s_a [cI w + cdd d (d.w)], w = v_1 - v_0 (masked)
+  const uint32_t m0 = mask[nd0], m1 = mask[nd1];
+  double w[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    w[j] = (((m1 >> j) & 1u) ? 0.0 : v[3 * (int64_t)nd1 + j]) -
+           (((m0 >> j) & 1u) ? 0.0 : v[3 * (int64_t)nd0 + j]);
+  const double dw = tv.d[0] * w[0] + tv.d[1] * w[1] + tv.d[2] * w[2];
+  const double sa = a ? 1.0 : -1.0;
+  const int32_t na = a ? nd1 : nd0;
+  const uint32_t ma = a ? m1 : m0;
+  double* yp = y + 3 * (int64_t)na;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) yp[i] += ((ma >> i) & 1u) ? 0.0 : sa * (tv.cI * w[i] + tv.cdd * tv.d[i] * dw);
+}
+
+// y[fixed] = v[fixed] (the identity rows of the assembled operator).
+__global__ void k_fixed_copy(const double* __restrict__ v, double* __restrict__ y,
+                             const uint8_t* __restrict__ mask, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const uint32_t m = mask[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if ((m >> d) & 1u) y[3 * i + d] = v[3 * i + d];
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -524,9 +607,8 @@ __global__ void k_fext(const double* __restrict__ x, const double* __restrict__ 
 // dJ/dG_e = -lambda_e . d f_e / d G_e with
 //   NeoHookean: dS/dG = I - C^-1  ->  d f_a / dG = V (F - F^-T) n_a
 //   SVK:        dS/dG = 2E        ->  d f_a / dG = 2 V F E n_a
-template <typename KT>
 __global__ void k_adjoint_shear(TetArgs ta, const double* __restrict__ U,
-                                const KT* __restrict__ lam, const int32_t* __restrict__ perm,
+                                const double* __restrict__ lam, const int32_t* __restrict__ perm,
                                 double* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ta.ne) return;
@@ -541,7 +623,7 @@ __global__ void k_adjoint_shear(TetArgs ta, const double* __restrict__ U,
     for (int a = 0; a < 4; ++a)
       for (int i = 0; i < 3; ++i) {
         double Fn = ev.F[i * 3] * n[a][0] + ev.F[i * 3 + 1] * n[a][1] + ev.F[i * 3 + 2] * n[a][2];
-        s += (double)lam[3 * (int64_t)nd[a] + i] * (Fn - ev.v[a][i]);
+        s += lam[3 * (int64_t)nd[a] + i] * (Fn - ev.v[a][i]);
       }
     out[perm[e]] = -V * s;
     return;
@@ -563,7 +645,7 @@ __global__ void k_adjoint_shear(TetArgs ta, const double* __restrict__ U,
   for (int a = 0; a < 4; ++a)
     for (int i = 0; i < 3; ++i) {
       double Dn = D[i * 3] * n[a][0] + D[i * 3 + 1] * n[a][1] + D[i * 3 + 2] * n[a][2];
-      s += (double)lam[3 * (int64_t)nd[a] + i] * Dn;
+      s += lam[3 * (int64_t)nd[a] + i] * Dn;
     }
   out[perm[e]] = -V * s;
 }
@@ -638,6 +720,30 @@ int launch_assemble(Ctx* c) {
   return c->precision == ONSAS_PRECISION_FP64 ? assemble_t<double>(c) : assemble_t<float>(c);
 }
 
+int launch_tangent_apply_f64(Ctx* c, const double* v, double* y) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_fill<double>, dim3(512), dim3(256), 0, s, y, 3 * c->nn, 0.0);
+  for (const ElemGroup& g : c->groups) {
+    const int ncol = (int)g.color_off.size() - 1;
+    for (int k = 0; k < ncol; ++k) {
+      const int64_t e0 = g.color_off[k], e1 = g.color_off[k + 1];
+      if (e1 <= e0) continue;
+      if (g.type == kTet)
+        hipLaunchKernelGGL(k_tet_apply, dim3(grid_for(4 * (e1 - e0), kScatterThreads)),
+                           dim3(kScatterThreads), 0, s, tet_args(g, e0, e1), c->d_U, v, y,
+                           c->d_fixmask);
+      else
+        hipLaunchKernelGGL(k_truss_apply, dim3(grid_for(2 * (e1 - e0), kScatterThreads)),
+                           dim3(kScatterThreads), 0, s, truss_args(g, e0, e1), c->d_U, v, y,
+                           c->d_fixmask);
+    }
+  }
+  hipLaunchKernelGGL(k_fixed_copy, dim3(grid_for(c->nn, 256)), dim3(256), 0, s, v, y, c->d_fixmask,
+                     c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
 int launch_element_outputs(Ctx* c, int gi, const double* U, double* f, double* K, double* stress,
                            double* strain) {
   const ElemGroup& g = c->groups[gi];
@@ -659,14 +765,10 @@ int launch_fext(Ctx* c) {
   return ONSAS_OK;
 }
 
-int launch_adjoint_shear(Ctx* c, int gi, const void* lam, double* out) {
+int launch_adjoint_shear(Ctx* c, int gi, const double* lam, double* out) {
   const ElemGroup& g = c->groups[gi];
-  if (c->precision == ONSAS_PRECISION_FP64)
-    hipLaunchKernelGGL(k_adjoint_shear<double>, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
-                       tet_args(g, 0, g.ne), c->d_U, (const double*)lam, g.d_perm, out);
-  else
-    hipLaunchKernelGGL(k_adjoint_shear<float>, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
-                       tet_args(g, 0, g.ne), c->d_U, (const float*)lam, g.d_perm, out);
+  hipLaunchKernelGGL(k_adjoint_shear, dim3(grid_for(g.ne, 128)), dim3(128), 0, c->stream,
+                     tet_args(g, 0, g.ne), c->d_U, lam, g.d_perm, out);
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
 }

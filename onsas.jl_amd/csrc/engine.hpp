@@ -193,7 +193,8 @@ int launch_assemble(Ctx* c);                                    // zero + scatte
 int launch_element_outputs(Ctx* c, int g, const double* U, double* f, double* K, double* stress,
                            double* strain);
 int launch_fext(Ctx* c);                                        // nodal + pressure faces
-int launch_adjoint_shear(Ctx* c, int g, const void* lambda_kt, double* dJdG_dev);
+int launch_adjoint_shear(Ctx* c, int g, const double* lambda_dev, double* dJdG_dev);
+int launch_tangent_apply_f64(Ctx* c, const double* v_dev, double* y_dev);  // y = K(U) v, FP64
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2
@@ -203,6 +204,9 @@ double linear_rtol(const Ctx* c);                               // configured / 
 int reset_forcing(Ctx* c);                                      // new load step
 int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged);
 int launch_update(Ctx* c);                                      // U += x, partial DU2/U2
+int launch_refine_residual(Ctx* c, const double* b_dev, const double* y_dev,
+                           double* r_dev);                      // r = b - y, partial R2/F2
+int launch_accumulate_x(Ctx* c, double* lam_dev);               // lam += PCG x (free dofs)
 int launch_finalize_newton(Ctx* c);                             // partials -> d_sc->nwt
 int launch_matvec(Ctx* c, const double* x_dev, double* y_dev);
 int launch_pack_sell(Ctx* c);                                  // block-CSR -> sliced-ELL copy

@@ -723,6 +723,50 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
   return ONSAS_OK;
 }
 
+// ---- iterative refinement of a solve: r = b - y on free dofs, ||r||^2, ||b||^2 -
+__global__ __launch_bounds__(NT) void k_refine_residual(const double* __restrict__ b,
+                                                          const double* __restrict__ y,
+                                                          const uint8_t* __restrict__ mask,
+                                                          double* __restrict__ r,
+                                                          double* __restrict__ pr2,
+                                                          double* __restrict__ pb2, int np,
+                                                          int64_t chunk, int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double r2 = 0, b2 = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    const uint32_t m = mask[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const bool fx = (m >> d) & 1u;
+      const double bv = fx ? 0.0 : b[3 * i + d];
+      const double rv = fx ? 0.0 : bv - y[3 * i + d];
+      r[3 * i + d] = rv;
+      r2 += rv * rv;
+      b2 += bv * bv;
+    }
+  }
+  r2 = block_sum(r2, sh);
+  b2 = block_sum(b2, sh);
+  if (threadIdx.x == 0) {
+    pr2[blockIdx.x] = r2;
+    pb2[blockIdx.x] = b2;
+  }
+}
+
+// lam[free] += x (the correction of one refinement pass)
+template <typename KT>
+__global__ void k_accumulate(const KT* __restrict__ x, const uint8_t* __restrict__ mask,
+                             double* __restrict__ lam, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const uint32_t m = mask[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (!((m >> d) & 1u)) lam[3 * i + d] += (double)x[3 * i + d];
+}
+
 template <typename KT>
 int update_t(Ctx* c) {
   hipLaunchKernelGGL(k_update<KT>, dim3(c->np), dim3(NT), 0, c->stream, (const KT*)c->d_px,
@@ -760,6 +804,26 @@ int reset_forcing(Ctx* c) {
 
 int launch_update(Ctx* c) {
   return c->precision == ONSAS_PRECISION_FP64 ? update_t<double>(c) : update_t<float>(c);
+}
+
+int launch_refine_residual(Ctx* c, const double* b_dev, const double* y_dev, double* r_dev) {
+  hipLaunchKernelGGL(k_refine_residual, dim3(c->np), dim3(NT), 0, c->stream, b_dev, y_dev,
+                     c->d_fixmask, r_dev, part(c, kPartR2), part(c, kPartF2), c->np, c->chunk,
+                     c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_accumulate_x(Ctx* c, double* lam_dev) {
+  const unsigned grid = grid_for(c->nn, 256);
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_accumulate<double>, dim3(grid), dim3(256), 0, c->stream,
+                       (const double*)c->d_px, c->d_fixmask, lam_dev, c->nn);
+  else
+    hipLaunchKernelGGL(k_accumulate<float>, dim3(grid), dim3(256), 0, c->stream,
+                       (const float*)c->d_px, c->d_fixmask, lam_dev, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
 }
 
 int launch_finalize_newton(Ctx* c) {

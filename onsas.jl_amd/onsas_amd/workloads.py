@@ -91,6 +91,39 @@ def truss_lattice(n=40, h=0.1, E=210e9, nu=0.3, A=1e-4, jitter=0.01, seed=1, F=N
     return s, sm
 
 
+def shear_field(points, G0=3.8e5, amp=0.3, ell=0.2, modes=256, seed=3):
+    """G(x) = G0 exp(amp g(x)) at the given points, with g a unit-variance
+    Gaussian field of squared-exponential correlation length ell, sampled by
+    `modes` random Fourier features: g = sqrt(2/M) sum cos(w_m . x + phi_m),
+    w_m ~ N(0, I / ell^2), phi_m ~ U(0, 2 pi)."""
+    rng = np.random.default_rng(seed)
+    w = rng.normal(size=(modes, 3)) / ell
+    phi = rng.uniform(0.0, 2 * np.pi, modes)
+    pts = np.asarray(points, dtype=np.float64)
+    g = np.empty(len(pts))
+    for i in range(0, len(pts), 65536):
+        g[i:i + 65536] = np.cos(pts[i:i + 65536] @ w.T + phi).sum(axis=1)
+    return G0 * np.exp(amp * np.sqrt(2.0 / modes) * g)
+
+
+def identification_cube(n=48, K=8.3e5, G0=3.8e5, alpha_final=0.85, jitter=0.02, seed=3,
+                        n_obs=2000, obs_seed=4):
+    """NeoHookean cube in compression with a heterogeneous per-element shear
+    modulus G_e = shear_field(centroid_e) (the parameters of an identification
+    problem), plus `n_obs` observed nodes drawn from the free surfaces
+    (x = 1, y = 1, z = 1).  The pressure is the homogeneous-G0 value of
+    cube_compression.  Returns (structure, mesh, p, G_true, obs_nodes)."""
+    sm = meshes.cube(n, jitter=jitter, seed=2)
+    cent = sm.coords[sm.tets].mean(axis=1)
+    G_true = shear_field(cent, G0=G0, seed=seed)
+    s, sm, p = cube_compression(n=n, K=K, G=G0, alpha_final=alpha_final, jitter=jitter, seed=2,
+                                G_field=G_true)
+    surf = np.unique(np.concatenate([sm.node_sets[k] for k in ("x1", "y1", "z1")]))
+    rng = np.random.default_rng(obs_seed)
+    obs = np.sort(rng.choice(surf, size=min(n_obs, len(surf)), replace=False))
+    return s, sm, p, G_true, obs
+
+
 # ---- closed forms ----------------------------------------------------------------
 
 def svk_uniaxial_alpha(E, p):
