@@ -1,6 +1,8 @@
 """LinearStaticAnalysis on the engine (src/StructuralAnalyses/LinearStaticAnalyses.jl:
 25-153): K assembled at the first load step, U = K^-1 f_ext per step (same
-block-Jacobi PCG), a re-assembly per step for the stresses and strains.
+block-Jacobi PCG), a re-assembly per step for the stresses and strains.  In
+mixed mode the FP32-tangent solve is followed by refinement steps against the
+FP64 residual until ||f_ext - f_int(U)|| <= 1e-10 ||f_ext||.
 """
 from __future__ import annotations
 
@@ -10,6 +12,9 @@ import numpy as np
 
 from .analysis import ΔU_and_ResidualForce_Criteria, ResidualsIterationStep, Solution, store_
 from .backend import GpuStaticState
+
+REFINE_TOL = 1e-10  # relative residual the mixed mode refines to
+REFINE_MAX = 4
 
 
 class LinearStaticAnalysis:
@@ -67,6 +72,18 @@ def _solve_linear_(sa: LinearStaticAnalysis, backend):
         ctx.set_U(U)
         sol.newton_log.append({"load_step": step, "lin_iters": it, "lin_rel_residual": rr})
         ctx.assemble()
+        if backend.precision == "mixed":
+            # the FP32 tangent solve is refined against the FP64 residual
+            # f_ext - f_int(U) (f_int = K U in FP64 for a linear material) until
+            # it meets the FP64 solve's accuracy
+            for _ in range(REFINE_MAX):
+                stats = ctx.newton_step()
+                ctx.assemble()
+                d = stats.as_dict()
+                d["load_step"] = step
+                sol.newton_log.append(d)
+                if stats.r_rel <= REFINE_TOL or stats.fext_norm == 0:
+                    break
         store_(sol, sa, step)
         sa.next_()
     sol.wall_s = time.perf_counter() - t0
