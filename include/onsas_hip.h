@@ -91,8 +91,9 @@ typedef struct {
 
 /* Cumulative device-time counters (hipEvents), reset by onsas_reset_timings. */
 typedef struct {
-    double assemble_ms;      /* zero + element scatter + block-Jacobi                */
-    double scatter_ms;       /* element kernels only                                 */
+    double assemble_ms;      /* whole assembly: f_int, K, block-Jacobi inverse       */
+    double scatter_ms;       /* element + block kernels (gather: state + gather pass; */
+                             /* scatter: the colour launches)                        */
     double solve_ms;         /* residual + linear solve + update                     */
     double pcg_ms;           /* PCG iterations only                                  */
     int64_t n_assemble;
@@ -101,6 +102,8 @@ typedef struct {
     int64_t gmres_iters;
     double spmv_ms;          /* event-timed SpMV kernel launches (first chunk of each PCG) */
     int64_t spmv_launches;   /* number of launches in spmv_ms                          */
+    double elem_ms;          /* gather assembly: the per-element state pass alone     */
+    int64_t assembly_mode;   /* 1 gather (default), 0 coloured scatter                */
 } onsas_timings;
 
 /* ---- lifetime ------------------------------------------------------------ */

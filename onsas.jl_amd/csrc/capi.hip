@@ -170,7 +170,7 @@ int assemble_now(Ctx* c) {
   ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_Uasm, c->d_U, 3 * c->nn * sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
   ONSAS_TRY(onsas::launch_assemble(c));
-  ONSAS_TRY(onsas::launch_pack_sell(c));
+  if (!c->gather) ONSAS_TRY(onsas::launch_pack_sell(c));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   return ONSAS_OK;
 }
@@ -223,13 +223,15 @@ void onsas_destroy(onsas_ctx* ctx) {
   (void)hipStreamSynchronize(c->stream);
   if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
   for (auto& g : c->groups)
-    for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm})
+    for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm,
+                    g.d_state, (void*)g.d_fe, (void*)g.d_cidx})
       if (p) (void)hipFree(p);
   void* ptrs[] = {c->d_x, c->d_y, c->d_z, c->d_fixmask, c->d_row_ptr, c->d_col, c->d_diag, c->d_K,
                   c->d_Dinv, c->d_U, c->d_Uasm, c->d_fint, c->d_fext, c->d_res, c->d_nodal, c->d_px, c->d_pr,
                   c->d_pz, c->d_pp0, c->d_pp1, c->d_pq, c->d_part, c->d_sc, c->d_face_conn,
                   c->d_face_grp, c->d_nf_ptr, c->d_nf_idx, c->d_pressures, c->d_gm_V, c->d_gm_w,
-                  c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds};
+                  c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds,
+                  c->d_contrib, c->d_sell_groups};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -383,6 +385,8 @@ int onsas_assemble(onsas_ctx* ctx, double* fint_out) {
   c->last_assemble_ms = elapsed(c->ev[0], c->ev[3]);
   c->tm.assemble_ms += c->last_assemble_ms;
   c->tm.scatter_ms += elapsed(c->ev[1], c->ev[2]);
+  if (c->gather) c->tm.elem_ms += elapsed(c->ev[1], c->ev[8]);
+  c->tm.assembly_mode = c->gather ? 1 : 0;
   c->tm.n_assemble++;
   c->assembled = true;
   return ONSAS_OK;

@@ -232,6 +232,7 @@ struct TetArgs {
   const double* geo;
   const int32_t* slot;
   const double* G;
+  const int32_t* idx;  // colour-sorted position -> array position (null: identity)
   int64_t ne, e0, e1;
   int mat;
   double p0, p1;
@@ -266,9 +267,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_tet_scatter(TetArgs ta,
                                                                   KT* __restrict__ K,
                                                                   const uint8_t* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
-  const int64_t e = ta.e0 + (t >> 2);
+  const int64_t k = ta.e0 + (t >> 2);
   const int a = (int)(t & 3);
-  if (e >= ta.e1) return;
+  if (k >= ta.e1) return;
+  const int64_t e = ta.idx ? (int64_t)ta.idx[k] : k;
   int32_t nd[4];
   double n[4][3], u[4][3], V, p1;
   tet_load(ta, e, U, nd, n, V, u, p1);
@@ -304,6 +306,7 @@ struct TrussArgs {
   const int32_t* conn;
   const double* geo;  // d0 (3), l0, A
   const int32_t* slot;
+  const int32_t* idx;  // colour-sorted position -> array position (null: identity)
   int64_t ne, e0, e1;
   int strain;
   double E;
@@ -316,9 +319,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_truss_scatter(TrussArgs ta,
                                                                     KT* __restrict__ K,
                                                                     const uint8_t* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
-  const int64_t e = ta.e0 + (t >> 1);
+  const int64_t k = ta.e0 + (t >> 1);
   const int a = (int)(t & 1);
-  if (e >= ta.e1) return;
+  if (k >= ta.e1) return;
+  const int64_t e = ta.idx ? (int64_t)ta.idx[k] : k;
   const int64_t ne = ta.ne;
   const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
   double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
@@ -354,6 +358,309 @@ __global__ __launch_bounds__(kScatterThreads) void k_truss_scatter(TrussArgs ta,
   }
 }
 
+// ---- gather assembly ---------------------------------------------------------
+// Pass 1 (one lane per element, coalesced SoA): the FP64 nodal forces f_a and
+// the factors of the tangent blocks
+//   K_ab = Vc1 v_a v_b' + Vc2 v_b v_a' + g_ab I (+ h_ab M for SVK),
+//   g_ab = V (c2 s_ab + sigma_ab) (NeoHookean, linear; M = I there),
+//   g_ab = V sigma_ab, h_ab = V c2 s_ab (SVK),
+// rounded once to KT.  Pass 2 (one lane per stored tangent block) sums the
+// blocks listed for it and writes it once.
+__device__ inline int pair_index(int a, int b) {  // a <= b over 4 nodes -> 0..9
+  return a * 4 - a * (a - 1) / 2 + (b - a);
+}
+
+template <typename KT>
+__global__ __launch_bounds__(256) void k_tet_state(TetArgs ta, const double* __restrict__ U,
+                                                   KT* __restrict__ st, double* __restrict__ fe,
+                                                   int hasM) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ta.ne) return;
+  const int64_t ne = ta.ne;
+  int32_t nd[4];
+  double n[4][3], u[4][3], V, p1;
+  tet_load(ta, e, U, nd, n, V, u, p1);
+  TetEval ev;
+  tet_eval(ta.mat, ta.p0, p1, n, u, ev);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double fa[3];
+    tet_force_row(ev, n, V, a, fa);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fe[(a * 3 + d) * ne + e] = fa[d];
+  }
+  st[e] = (KT)(V * ev.c1);
+  st[ne + e] = (KT)(V * ev.c2);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) st[(2 + 3 * a + i) * ne + e] = (KT)ev.v[a][i];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b) {
+      const int p = pair_index(a, b);
+      const double s = ev.sv ? ev.v[a][0] * ev.v[b][0] + ev.v[a][1] * ev.v[b][1] + ev.v[a][2] * ev.v[b][2]
+                             : n[a][0] * n[b][0] + n[a][1] * n[b][1] + n[a][2] * n[b][2];
+      double sig = 0;
+      if (ev.geo) {
+        double Sn[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          Sn[i] = ev.S[i * 3] * n[b][0] + ev.S[i * 3 + 1] * n[b][1] + ev.S[i * 3 + 2] * n[b][2];
+        sig = n[a][0] * Sn[0] + n[a][1] * Sn[1] + n[a][2] * Sn[2];
+      }
+      if (hasM) {
+        st[(14 + p) * ne + e] = (KT)(V * sig);
+        st[(24 + p) * ne + e] = (KT)(V * ev.c2 * s);
+      } else {
+        st[(14 + p) * ne + e] = (KT)(V * (ev.c2 * s + sig));
+      }
+    }
+  if (hasM) {
+    st[34 * ne + e] = (KT)ev.M[0];
+    st[35 * ne + e] = (KT)ev.M[1];
+    st[36 * ne + e] = (KT)ev.M[2];
+    st[37 * ne + e] = (KT)ev.M[4];
+    st[38 * ne + e] = (KT)ev.M[5];
+    st[39 * ne + e] = (KT)ev.M[8];
+  }
+}
+
+template <typename KT>
+__global__ __launch_bounds__(256) void k_truss_state(TrussArgs ta, const double* __restrict__ U,
+                                                     KT* __restrict__ st, double* __restrict__ fe) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ta.ne) return;
+  const int64_t ne = ta.ne;
+  const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
+  double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
+  const double l0 = ta.geo[3 * ne + e], A = ta.geo[4 * ne + e];
+  double du[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) du[d] = U[3 * (int64_t)nd1 + d] - U[3 * (int64_t)nd0 + d];
+  TrussEval tv;
+  truss_eval(ta.strain, ta.E, A, d0, l0, du, tv);
+  st[e] = (KT)tv.cdd;
+  st[ne + e] = (KT)tv.cI;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    st[(2 + d) * ne + e] = (KT)tv.d[d];
+    fe[d * ne + e] = tv.cf * tv.d[d];  // f_1 = +cf d, f_0 = -cf d
+  }
+}
+
+struct GatherGroup {
+  const void* state;
+  const double* fe;
+  int64_t ne, goff;
+  int type, hasM;
+};
+
+struct GatherArgs {
+  GatherGroup g[kMaxGatherGroups];
+  int ng;
+};
+
+constexpr uint32_t kNoContrib = 0xffffffffu;
+
+// One element block (a, b) in factored form, loaded from the state planes.
+// w = 0 marks a padding entry (the loads then read element 0 of group 0 and
+// the block adds nothing), so the loop body has no branches on the entry.
+template <typename KT, bool HM>
+struct Contrib {
+  KT w, c1, c2, g, va[3], vb[3];
+  KT h, M[HM ? 6 : 1];
+  double f[3];
+  int tet, hasM;
+};
+
+// HM: some group of the model carries the SVK M term (compiled out otherwise)
+template <typename KT, bool HM>
+__device__ inline void load_contrib(const GatherArgs& ga, uint32_t en, bool dg, Contrib<KT, HM>& c) {
+  const bool valid = en != kNoContrib;
+  const uint32_t ev = valid ? en : 0u;
+  const int64_t ge = ev >> 4;
+  const int a = (ev >> 2) & 3, b = ev & 3;
+  const void* sp = ga.g[0].state;
+  const double* fp = ga.g[0].fe;
+  int64_t ne = ga.g[0].ne, goff = ga.g[0].goff;
+  int type = ga.g[0].type, hasM = ga.g[0].hasM;
+#pragma unroll
+  for (int j = 1; j < kMaxGatherGroups; ++j)
+    if (j < ga.ng && ge >= ga.g[j].goff) {
+      sp = ga.g[j].state;
+      fp = ga.g[j].fe;
+      ne = ga.g[j].ne;
+      goff = ga.g[j].goff;
+      type = ga.g[j].type;
+      hasM = ga.g[j].hasM;
+    }
+  const int64_t e = ge - goff;
+  const KT* st = (const KT*)sp;
+  c.w = valid ? KT(1) : KT(0);
+  c.tet = type == kTet;
+  c.hasM = hasM;
+  c.c1 = st[e];
+  c.c2 = st[ne + e];
+  c.h = KT(0);
+  if (c.tet) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      c.va[i] = st[(2 + 3 * a + i) * ne + e];
+      c.vb[i] = st[(2 + 3 * b + i) * ne + e];
+    }
+    const int p = a <= b ? pair_index(a, b) : pair_index(b, a);
+    c.g = st[(14 + p) * ne + e];
+    if (HM && hasM) {
+      c.h = st[(24 + p) * ne + e];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) c.M[q] = st[(34 + q) * ne + e];
+    }
+    if (dg) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) c.f[d] = fp[(3 * a + d) * ne + e];
+    }
+  } else {
+    // truss: K_ab = s_a s_b [cI I + cdd d d'] in the same factored form
+    const KT sab = (a == b) ? KT(1) : KT(-1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      c.va[i] = st[(2 + i) * ne + e];
+      c.vb[i] = sab * c.va[i];
+    }
+    c.g = sab * c.c2;  // c2 plane holds cI
+    c.c2 = KT(0);
+    if (dg) {
+      const double sa = a ? 1.0 : -1.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) c.f[d] = sa * fp[d * ne + e];
+    }
+  }
+  if (!dg || !valid) c.f[0] = c.f[1] = c.f[2] = 0.0;
+}
+
+template <typename KT, bool HM>
+__device__ inline void add_contrib(const Contrib<KT, HM>& c, KT acc[9], double& f0, double& f1,
+                                   double& f2) {
+  const KT c1 = c.w * c.c1, c2 = c.w * c.c2, g = c.w * c.g;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[i * 3 + j] += c1 * c.va[i] * c.vb[j] + c2 * c.vb[i] * c.va[j];
+  acc[0] += g;
+  acc[4] += g;
+  acc[8] += g;
+  if (HM && c.hasM) {
+    const KT h = c.w * c.h;
+    acc[0] += h * c.M[0];
+    acc[1] += h * c.M[1];
+    acc[2] += h * c.M[2];
+    acc[3] += h * c.M[1];
+    acc[4] += h * c.M[3];
+    acc[5] += h * c.M[4];
+    acc[6] += h * c.M[2];
+    acc[7] += h * c.M[4];
+    acc[8] += h * c.M[5];
+  }
+  f0 += c.f[0];
+  f1 += c.f[1];
+  f2 += c.f[2];
+}
+
+// One wave per sliced-ELL group (64 rows at one round), one lane per block.
+// The contribution rounds are uniform over the wave (padded lists), the
+// element-group lookup is a predicated select (no dynamic indexing of the
+// kernel arguments).  The lane holding a diagonal block also writes the row's
+// f_int (its contributions are exactly the node's element incidences) and the
+// FP64-computed inverse of the finished block (block-Jacobi).  Fixed dofs:
+// their rows and columns are zeroed and the diagonal entry set to 1, as the
+// scatter path does.
+template <typename KT, bool HM>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs ga, const SellGroup* __restrict__ groups,
+                                                int64_t ngroups, const uint32_t* __restrict__ contrib,
+                                                const int32_t* __restrict__ row_ptr,
+                                                const int32_t* __restrict__ col,
+                                                const uint8_t* __restrict__ mask,
+                                                KT* __restrict__ Ks, KT* __restrict__ Dinv,
+                                                double* __restrict__ fint, int64_t nn) {
+  // XCD-aware: hardware workgroup ids go round-robin over the 8 XCDs; logical
+  // block (b % 8) * (nb / 8) + b / 8 gives each XCD a contiguous run of slices,
+  // so the element states a slice's rounds share stay in one L2
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t L = (b & 7) * (nb >> 3) + (b >> 3);
+  const int64_t w = L * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= ngroups) return;
+  const SellGroup sg = groups[w];
+  const int64_t r = sg.row0 + lane % kSellRows;
+  const int k = kSellLanes * sg.round + lane / kSellRows;
+  if (r >= nn) return;
+  const int32_t rs = row_ptr[r];
+  if (k >= row_ptr[r + 1] - rs) return;
+  const int32_t cc = col[rs + k];
+  const bool dg = (cc == r);
+  KT acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = KT(0);
+  double f0 = 0, f1 = 0, f2 = 0;
+  const uint32_t* cp = contrib + sg.cbase + lane;
+  const int qw = sg.qwidth;
+  // two contributions per step; the entries of the next step are loaded before
+  // this step's element states, so each step waits on one memory latency
+  uint32_t n0 = qw > 0 ? cp[0] : kNoContrib;
+  uint32_t n1 = qw > 1 ? cp[64] : kNoContrib;
+  for (int q = 0; q < qw; q += 2) {
+    const uint32_t e0 = n0, e1 = n1;
+    n0 = q + 2 < qw ? cp[64 * (int64_t)(q + 2)] : kNoContrib;
+    n1 = q + 3 < qw ? cp[64 * (int64_t)(q + 3)] : kNoContrib;
+    Contrib<KT, HM> c0, c1;
+    load_contrib(ga, e0, dg, c0);
+    load_contrib(ga, e1, dg, c1);
+    add_contrib(c0, acc, f0, f1, f2);
+    add_contrib(c1, acc, f0, f1, f2);
+  }
+  const uint32_t mr = mask[r], mc = mask[cc];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (((mr >> i) & 1u) | ((mc >> j) & 1u)) acc[i * 3 + j] = (dg && i == j) ? KT(1) : KT(0);
+  KT* o = Ks + 9 * (int64_t)sg.base + lane;
+#pragma unroll
+  for (int v = 0; v < 9; ++v) o[64 * v] = acc[v];
+  if (dg) {
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[q] = (double)acc[q];
+    const double c0 = A[4] * A[8] - A[5] * A[7], cf1 = A[5] * A[6] - A[3] * A[8],
+                 cf2 = A[3] * A[7] - A[4] * A[6];
+    const double rdet = 1.0 / (A[0] * c0 + A[1] * cf1 + A[2] * cf2);
+    KT* di = Dinv + 9 * r;
+    di[0] = (KT)(c0 * rdet);
+    di[1] = (KT)((A[2] * A[7] - A[1] * A[8]) * rdet);
+    di[2] = (KT)((A[1] * A[5] - A[2] * A[4]) * rdet);
+    di[3] = (KT)(cf1 * rdet);
+    di[4] = (KT)((A[0] * A[8] - A[2] * A[6]) * rdet);
+    di[5] = (KT)((A[2] * A[3] - A[0] * A[5]) * rdet);
+    di[6] = (KT)(cf2 * rdet);
+    di[7] = (KT)((A[1] * A[6] - A[0] * A[7]) * rdet);
+    di[8] = (KT)((A[0] * A[4] - A[1] * A[3]) * rdet);
+    fint[3 * r] = f0;
+    fint[3 * r + 1] = f1;
+    fint[3 * r + 2] = f2;
+  }
+}
+
+// Block-Jacobi rows of nodes without a diagonal block (fully fixed, no element).
+template <typename KT>
+__global__ void k_dinv_identity(const int32_t* __restrict__ diag, KT* __restrict__ Dinv, int64_t nn) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn || diag[i] >= 0) return;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Dinv[9 * i + q] = (q % 4) == 0 ? KT(1) : KT(0);
+}
+
 // y += K(U) v with the element blocks evaluated in FP64 and never stored: the
 // FP64 tangent product used to refine mixed-precision solves.  Lanes, colours
 // and the fixed-dof masking are those of k_tet_scatter, so the operator is the
@@ -364,9 +671,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_tet_apply(TetArgs ta,
                                                                double* __restrict__ y,
                                                                const uint8_t* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
-  const int64_t e = ta.e0 + (t >> 2);
+  const int64_t k = ta.e0 + (t >> 2);
   const int a = (int)(t & 3);
-  if (e >= ta.e1) return;
+  if (k >= ta.e1) return;
+  const int64_t e = ta.idx ? (int64_t)ta.idx[k] : k;
   int32_t nd[4];
   double n[4][3], u[4][3], V, p1;
   tet_load(ta, e, U, nd, n, V, u, p1);
@@ -398,9 +706,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_truss_apply(TrussArgs ta,
                                                                  double* __restrict__ y,
                                                                  const uint8_t* __restrict__ mask) {
   const int64_t t = (int64_t)blockIdx.x * kScatterThreads + threadIdx.x;
-  const int64_t e = ta.e0 + (t >> 1);
+  const int64_t k = ta.e0 + (t >> 1);
   const int a = (int)(t & 1);
-  if (e >= ta.e1) return;
+  if (k >= ta.e1) return;
+  const int64_t e = ta.idx ? (int64_t)ta.idx[k] : k;
   const int64_t ne = ta.ne;
   const int32_t nd0 = ta.conn[e], nd1 = ta.conn[ne + e];
   double d0[3] = {ta.geo[e], ta.geo[ne + e], ta.geo[2 * ne + e]};
@@ -656,6 +965,7 @@ TetArgs tet_args(const ElemGroup& g, int64_t e0, int64_t e1) {
   ta.geo = g.d_geo;
   ta.slot = g.d_slot;
   ta.G = g.d_G;
+  ta.idx = g.d_cidx;
   ta.ne = g.ne;
   ta.e0 = e0;
   ta.e1 = e1;
@@ -670,6 +980,7 @@ TrussArgs truss_args(const ElemGroup& g, int64_t e0, int64_t e1) {
   ta.conn = g.d_conn;
   ta.geo = g.d_geo;
   ta.slot = g.d_slot;
+  ta.idx = g.d_cidx;
   ta.ne = g.ne;
   ta.e0 = e0;
   ta.e1 = e1;
@@ -714,10 +1025,61 @@ int assemble_t(Ctx* c) {
   return ONSAS_OK;
 }
 
+template <typename KT>
+int assemble_gather_t(Ctx* c) {
+  hipStream_t s = c->stream;
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  GatherArgs ga{};
+  ga.ng = (int)c->groups.size();
+  for (int gi = 0; gi < ga.ng; ++gi) {
+    const ElemGroup& g = c->groups[gi];
+    if (g.type == kTet)
+      hipLaunchKernelGGL(k_tet_state<KT>, dim3(grid_for(g.ne, 256)), dim3(256), 0, s,
+                         tet_args(g, 0, g.ne), c->d_U, (KT*)g.d_state, g.d_fe, (int)g.hasM);
+    else
+      hipLaunchKernelGGL(k_truss_state<KT>, dim3(grid_for(g.ne, 256)), dim3(256), 0, s,
+                         truss_args(g, 0, g.ne), c->d_U, (KT*)g.d_state, g.d_fe);
+    ga.g[gi] = GatherGroup{g.d_state, g.d_fe, g.ne, g.goff, g.type, (int)g.hasM};
+  }
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[8], s));
+  const int64_t ngrp = (int64_t)c->sell_groups.size();
+  const unsigned gb = (grid_for(ngrp, 4) + 7) / 8 * 8;  // a multiple of 8 (XCD mapping)
+  bool hm = false;
+  for (const ElemGroup& g : c->groups) hm = hm || g.hasM;
+  if (hm)
+    hipLaunchKernelGGL((k_gather<KT, true>), dim3(gb), dim3(256), 0, s, ga, c->d_sell_groups,
+                       ngrp, c->d_contrib, c->d_row_ptr, c->d_col, c->d_fixmask, (KT*)c->d_Ksell,
+                       (KT*)c->d_Dinv, c->d_fint, c->nn);
+  else
+    hipLaunchKernelGGL((k_gather<KT, false>), dim3(gb), dim3(256), 0, s, ga, c->d_sell_groups,
+                       ngrp, c->d_contrib, c->d_row_ptr, c->d_col, c->d_fixmask, (KT*)c->d_Ksell,
+                       (KT*)c->d_Dinv, c->d_fint, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev[2], s));
+  c->bsr_stale = true;
+  return ONSAS_OK;
+}
+
 }  // namespace
 
 int launch_assemble(Ctx* c) {
+  if (c->gather)
+    return c->precision == ONSAS_PRECISION_FP64 ? assemble_gather_t<double>(c)
+                                                : assemble_gather_t<float>(c);
   return c->precision == ONSAS_PRECISION_FP64 ? assemble_t<double>(c) : assemble_t<float>(c);
+}
+
+int launch_gather_init(Ctx* c) {
+  const unsigned grid = grid_for(c->nn, 256);
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_dinv_identity<double>, dim3(grid), dim3(256), 0, c->stream, c->d_diag,
+                       (double*)c->d_Dinv, c->nn);
+  else
+    hipLaunchKernelGGL(k_dinv_identity<float>, dim3(grid), dim3(256), 0, c->stream, c->d_diag,
+                       (float*)c->d_Dinv, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
 }
 
 int launch_tangent_apply_f64(Ctx* c, const double* v, double* y) {

@@ -64,6 +64,34 @@ struct ElemGroup {
   int32_t* d_slot = nullptr;
   double* d_G = nullptr;
   int32_t* d_perm = nullptr;
+  // gather mode: the arrays above are in gather order (see setup) and the
+  // colour launches reach them through cidx (colour-sorted position -> position)
+  std::vector<int32_t> cidx;
+  int32_t* d_cidx = nullptr;
+  // gather assembly: per-element tangent factors (KT planes) and nodal forces
+  // (FP64 planes), both SoA with stride ne in the sorted order
+  void* d_state = nullptr;
+  double* d_fe = nullptr;
+  int nstate = 0;
+  bool hasM = false;     // SVK: the c2 s_ab M term needs M = F F' (M = I otherwise)
+  int64_t goff = 0;      // first global element id of the group (contribution lists)
+};
+
+// Tet state planes (stride ne): 0 V c1, 1 V c2, 2..13 v[a][i], 14..23 g_ab (the
+// coefficient of I), SVK only: 24..33 h_ab (the coefficient of M), 34..39 M
+// (00 01 02 11 12 22).  Truss: 0 cdd, 1 cI, 2..4 d.  Pairs a <= b numbered
+// 00 01 02 03 11 12 13 22 23 33.
+constexpr int kTetStateBase = 24, kTetStateM = 40, kTrussState = 5;
+constexpr int kMaxGatherGroups = 8;
+
+// One sliced-ELL group = the 64 lanes of one slice at one round: the unit of
+// work (one wave) of the gather assembly.
+struct SellGroup {
+  int32_t base;    // block index of lane 0 in the sliced-ELL arrays
+  int32_t row0;    // first row of the slice
+  int32_t round;   // m
+  int32_t qwidth;  // contribution rounds (max over the lanes)
+  int64_t cbase;   // first contribution entry (entries of round q at cbase + 64 q + lane)
 };
 
 struct PcgState {     // device-resident scalars of the PCG iteration
@@ -134,6 +162,16 @@ struct Ctx {
   int32_t* d_sell_col = nullptr;
   void* d_Ksell = nullptr;
 
+  // assembly mode: gather (default) writes every sliced-ELL block once from the
+  // per-element states through fixed-order contribution lists; scatter adds the
+  // element blocks colour by colour into the block-CSR copy and packs it.
+  bool gather = true;
+  std::vector<SellGroup> sell_groups;
+  SellGroup* d_sell_groups = nullptr;
+  uint32_t* d_contrib = nullptr;  // (global element << 4) | (a << 2) | b, 0xffffffff = none
+  int64_t ncontrib = 0;           // stored entries including padding
+  bool bsr_stale = false;         // d_K does not hold the last assembly yet
+
   // device
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
   uint8_t* d_fixmask = nullptr;
@@ -171,7 +209,7 @@ struct Ctx {
   int gm_m = 0;
 
   // timers
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};  // 0/3 assembly, 1/8/2 element+gather passes, 4..7 Newton step
   hipEvent_t ev_spmv[2 * 32] = {};  // brackets of the sampled SpMV launches
   onsas_timings tm{};
   double last_assemble_ms = 0;
@@ -194,6 +232,8 @@ int launch_element_outputs(Ctx* c, int g, const double* U, double* f, double* K,
                            double* strain);
 int launch_fext(Ctx* c);                                        // nodal + pressure faces
 int launch_adjoint_shear(Ctx* c, int g, const double* lambda_dev, double* dJdG_dev);
+int launch_gather_init(Ctx* c);                                 // zero padding, identity Dinv
+int ensure_bsr(Ctx* c);                                         // d_K from d_Ksell if stale
 int launch_tangent_apply_f64(Ctx* c, const double* v_dev, double* y_dev);  // y = K(U) v, FP64
 
 // ---- solver_kernels.hip ----

@@ -89,6 +89,135 @@ int color_group(Ctx* c, ElemGroup& g) {
   return ONSAS_OK;
 }
 
+// Element order of the gather assembly.  Lane l of a gather wave handles row
+// r0 + l; for its q-th contribution to be next to lane l+1's in the state
+// planes, elements that are translates of each other must be adjacent.  Key:
+// the element's shape signature (node ids relative to its first node) and then
+// its first node: on a structured mesh every translate class is one run in
+// node order.  When the signatures do not repeat (unstructured meshes) the key
+// falls back to the smallest node id, which keeps the node-numbering locality.
+std::vector<int32_t> gather_order(const ElemGroup& g) {
+  const int nn_e = g.nnode;
+  const int64_t ne = g.ne;
+  std::vector<int32_t> order(ne);
+  std::iota(order.begin(), order.end(), 0);
+  auto sig_less = [&](int32_t x, int32_t y) {
+    const int32_t* cx = &g.conn[(size_t)x * nn_e];
+    const int32_t* cy = &g.conn[(size_t)y * nn_e];
+    for (int a = 1; a < nn_e; ++a) {
+      const int64_t dx = (int64_t)cx[a] - cx[0], dy = (int64_t)cy[a] - cy[0];
+      if (dx != dy) return dx < dy;
+    }
+    if (cx[0] != cy[0]) return cx[0] < cy[0];
+    return x < y;
+  };
+  std::sort(order.begin(), order.end(), sig_less);
+  int64_t classes = 1;
+  for (int64_t k = 1; k < ne; ++k) {
+    const int32_t* cx = &g.conn[(size_t)order[k - 1] * nn_e];
+    const int32_t* cy = &g.conn[(size_t)order[k] * nn_e];
+    for (int a = 1; a < nn_e; ++a)
+      if (cx[a] - cx[0] != cy[a] - cy[0]) {
+        ++classes;
+        break;
+      }
+  }
+  if (classes > 512) {
+    std::vector<int32_t> mn(ne);
+    for (int64_t e = 0; e < ne; ++e) {
+      const int32_t* ce = &g.conn[(size_t)e * nn_e];
+      mn[e] = *std::min_element(ce, ce + nn_e);
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return mn[x] < mn[y]; });
+  }
+  return order;
+}
+
+// Contribution lists of the gather assembly.  Block (r, k) of the tangent
+// receives the element blocks (e, a, b) whose slot is that block; they are
+// listed by (global element, a, b), so every assembly sums them in the same
+// order.  The lists are stored per sliced-ELL group, interleaved by lane
+// (entry q of lane l at cbase + 64 q + l) so that each contribution round is
+// one contiguous 256-byte load for the wave; short lists are padded with
+// 0xffffffff.  Also allocates the per-element state planes.
+int build_gather(Ctx* c, const std::vector<std::vector<int32_t>>& slots, int64_t rstride) {
+  const int64_t nnzb = c->nnzb, nn = c->nn;
+  std::vector<int64_t> ptr(nnzb + 1, 0);
+  for (size_t gi = 0; gi < c->groups.size(); ++gi)
+    for (int32_t blk : slots[gi]) ptr[blk + 1]++;
+  for (int64_t b = 0; b < nnzb; ++b) ptr[b + 1] += ptr[b];
+  std::vector<uint32_t> ent(ptr[nnzb]);
+  {
+    std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+    for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+      const ElemGroup& g = c->groups[gi];
+      const int nn_e = g.nnode;
+      for (int64_t e = 0; e < g.ne; ++e)
+        for (int a = 0; a < nn_e; ++a)
+          for (int b = 0; b < nn_e; ++b) {
+            const int32_t blk = slots[gi][(size_t)(a * nn_e + b) * g.ne + e];
+            ent[pos[blk]++] = (uint32_t)(((g.goff + e) << 4) | (a << 2) | b);
+          }
+    }
+  }
+  // sliced-ELL groups, slice-major: the rounds of one slice run side by side
+  // and share the element states of their rows in L2
+  c->sell_groups.clear();
+  auto add_group = [&](int64_t s, int m) {
+    SellGroup sg;
+    sg.base = (int32_t)(c->sell_ptr[s] + rstride * m);
+    sg.row0 = (int32_t)(kSellRows * s);
+    sg.round = m;
+    sg.qwidth = 0;
+    sg.cbase = 0;
+    for (int lane = 0; lane < 64; ++lane) {
+      const int64_t r = kSellRows * s + lane % kSellRows;
+      const int k = kSellLanes * m + lane / kSellRows;
+      if (r < nn && k < c->row_ptr[r + 1] - c->row_ptr[r]) {
+        const int64_t blk = c->row_ptr[r] + k;
+        sg.qwidth = std::max<int32_t>(sg.qwidth, (int32_t)(ptr[blk + 1] - ptr[blk]));
+      }
+    }
+    c->sell_groups.push_back(sg);
+  };
+  for (int64_t s = 0; s < c->nslices; ++s)
+    for (int m = 0; m < c->slice_rounds[s]; ++m) add_group(s, m);
+  int64_t total = 0;
+  for (SellGroup& sg : c->sell_groups) {
+    sg.cbase = total;
+    total += 64 * (int64_t)sg.qwidth;
+  }
+  c->ncontrib = total;
+  std::vector<uint32_t> lists(std::max<int64_t>(total, 1), 0xffffffffu);
+  for (const SellGroup& sg : c->sell_groups) {
+    const int64_t s = sg.row0 / kSellRows;
+    for (int lane = 0; lane < 64; ++lane) {
+      const int64_t r = kSellRows * s + lane % kSellRows;
+      const int k = kSellLanes * sg.round + lane / kSellRows;
+      if (r >= nn || k >= c->row_ptr[r + 1] - c->row_ptr[r]) continue;
+      const int64_t blk = c->row_ptr[r] + k;
+      for (int64_t q = ptr[blk]; q < ptr[blk + 1]; ++q)
+        lists[sg.cbase + 64 * (int64_t)(q - ptr[blk]) + lane] = ent[q];
+    }
+  }
+  ONSAS_TRY(dev_upload(c, &c->d_contrib, lists.data(), lists.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_sell_groups, c->sell_groups.data(), c->sell_groups.size()));
+  for (ElemGroup& g : c->groups) {
+    if (g.type == kTet) {
+      g.hasM = (g.mat == ONSAS_MAT_SVK);
+      g.nstate = g.hasM ? kTetStateM : kTetStateBase;
+    } else {
+      g.hasM = false;
+      g.nstate = kTrussState;
+    }
+    ONSAS_TRY(dev_alloc(c, &g.d_state, c->kt * g.nstate * (size_t)g.ne));
+    ONSAS_TRY(dev_alloc(c, (void**)&g.d_fe,
+                        sizeof(double) * (g.type == kTet ? 12 : 3) * (size_t)g.ne));
+  }
+  return ONSAS_OK;
+}
+
 }  // namespace
 
 int finalize_model(Ctx* c) {
@@ -104,8 +233,28 @@ int finalize_model(Ctx* c) {
   for (uint8_t m : c->fixmask) nfixed += __builtin_popcount(m);
   c->nfree = n - nfixed;
 
-  // colouring
-  for (auto& g : c->groups) ONSAS_TRY(color_group(c, g));
+  // assembly mode (gather unless asked otherwise or out of its limits)
+  {
+    const char* am = std::getenv("ONSAS_ASSEMBLY");
+    int64_t ntot = 0;
+    for (const ElemGroup& g : c->groups) ntot += g.ne;
+    c->gather = !(am && std::string(am) == "scatter") &&
+                (int)c->groups.size() <= kMaxGatherGroups && ntot < (int64_t(1) << 28);
+  }
+
+  // colouring; in gather mode the arrays take the gather order and the colour
+  // launches index them through cidx
+  for (auto& g : c->groups) {
+    ONSAS_TRY(color_group(c, g));
+    if (c->gather) {
+      std::vector<int32_t> order = gather_order(g);
+      std::vector<int32_t> pos(g.ne);
+      for (int64_t k = 0; k < g.ne; ++k) pos[order[k]] = (int32_t)k;
+      g.cidx.resize(g.ne);
+      for (int64_t k = 0; k < g.ne; ++k) g.cidx[k] = pos[g.perm[k]];
+      g.perm = std::move(order);
+    }
+  }
 
   // block pattern: every (node_a, node_b) pair of every element
   std::vector<int64_t> cnt(nn + 1, 0);
@@ -154,12 +303,19 @@ int finalize_model(Ctx* c) {
     return (int32_t)(std::lower_bound(b, e, cc) - c->col.begin());
   };
 
+  std::vector<std::vector<int32_t>> slots_h(c->groups.size());
+
   // per-group device arrays in colour-sorted order
+  int64_t goff = 0;
   for (size_t gi = 0; gi < c->groups.size(); ++gi) {
     ElemGroup& g = c->groups[gi];
     const int64_t ne = g.ne;
     const int nn_e = g.nnode;
-    std::vector<int32_t> conn((size_t)nn_e * ne), slot((size_t)nn_e * nn_e * ne);
+    g.goff = goff;
+    goff += ne;
+    std::vector<int32_t> conn((size_t)nn_e * ne);
+    std::vector<int32_t>& slot = slots_h[gi];
+    slot.resize((size_t)nn_e * nn_e * ne);
     int ngeo = (g.type == kTet) ? 13 : 5;
     std::vector<double> geo((size_t)ngeo * ne);
     std::vector<double> Gs;
@@ -199,7 +355,9 @@ int finalize_model(Ctx* c) {
     ONSAS_TRY(dev_upload(c, &g.d_slot, slot.data(), slot.size()));
     ONSAS_TRY(dev_upload(c, &g.d_geo, geo.data(), geo.size()));
     ONSAS_TRY(dev_upload(c, &g.d_perm, g.perm.data(), g.perm.size()));
+    if (!g.cidx.empty()) ONSAS_TRY(dev_upload(c, &g.d_cidx, g.cidx.data(), g.cidx.size()));
     if (g.per_elem_G) ONSAS_TRY(dev_upload(c, &g.d_G, Gs.data(), Gs.size()));
+    if (!c->gather) slot.clear();
   }
 
   // nodes, pattern, masks
@@ -303,7 +461,9 @@ int finalize_model(Ctx* c) {
     ONSAS_TRY(dev_upload(c, &c->d_sell_col, scol.data(), scol.size()));
     ONSAS_TRY(dev_alloc(c, &c->d_Ksell, c->kt * 9 * (size_t)c->nsell));
   }
+  if (c->gather) ONSAS_TRY(build_gather(c, slots_h, rstride));
 
+  if (c->gather) ONSAS_TRY(launch_gather_init(c));
   for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
   for (auto& e : c->ev_spmv) ONSAS_HIP_TRY(c, hipEventCreate(&e));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));

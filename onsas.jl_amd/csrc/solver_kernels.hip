@@ -296,6 +296,33 @@ __global__ __launch_bounds__(256) void k_pack_sell(const int32_t* __restrict__ r
   }
 }
 
+// The inverse of k_pack_sell: sliced-ELL -> block-CSR (for the exports and the
+// block-CSR matvec after a gather assembly, which writes only the sliced-ELL copy).
+template <typename KT>
+__global__ __launch_bounds__(256) void k_unpack_sell(const int32_t* __restrict__ row_ptr,
+                                                       const KT* __restrict__ Ks,
+                                                       const int32_t* __restrict__ sell_ptr,
+                                                       const int32_t* __restrict__ slice_rounds,
+                                                       int64_t rstride, KT* __restrict__ K,
+                                                       int64_t nslices, int64_t nn) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nslices) return;
+  const int64_t r = kSellRows * s + lane % kSellRows;
+  const int ph = lane / kSellRows;
+  if (r >= nn) return;
+  const int32_t b0 = sell_ptr[s], rounds = slice_rounds[s];
+  const int32_t rs = row_ptr[r], len = row_ptr[r + 1] - rs;
+  for (int m = 0; m < rounds; ++m) {
+    const int k = kSellLanes * m + ph;
+    if (k >= len) continue;
+    const KT* src = Ks + 9 * (b0 + rstride * m) + lane;
+    KT* dst = K + 9 * (int64_t)(rs + k);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) dst[v] = src[64 * v];
+  }
+}
+
 // pcg_A on the sliced-ELL tangent: one wave per slice of kSellRows block rows,
 // kSellLanes lanes (slot phases) per row combined by one shuffle stage.
 // Workgroup w owns rows [r0, r0 + chunk) (whole slices); its waves stride over
@@ -833,6 +860,7 @@ int launch_finalize_newton(Ctx* c) {
 }
 
 int launch_matvec(Ctx* c, const double* x_dev, double* y_dev) {
+  ONSAS_TRY(ensure_bsr(c));
   const unsigned grid = grid_for(c->nn * kRowGroup, 256);
   if (c->precision == ONSAS_PRECISION_FP64)
     hipLaunchKernelGGL(k_spmv_f64<double>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
@@ -859,6 +887,23 @@ int launch_pack_sell(Ctx* c) {
   return ONSAS_OK;
 }
 
+int ensure_bsr(Ctx* c) {
+  if (!c->bsr_stale) return ONSAS_OK;
+  const unsigned grid = grid_for(c->nslices, 4);
+  const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
+  if (c->precision == ONSAS_PRECISION_FP64)
+    hipLaunchKernelGGL(k_unpack_sell<double>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       (const double*)c->d_Ksell, c->d_sell_ptr, c->d_slice_rounds, rstride,
+                       (double*)c->d_K, c->nslices, c->nn);
+  else
+    hipLaunchKernelGGL(k_unpack_sell<float>, dim3(grid), dim3(256), 0, c->stream, c->d_row_ptr,
+                       (const float*)c->d_Ksell, c->d_sell_ptr, c->d_slice_rounds, rstride,
+                       (float*)c->d_K, c->nslices, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  c->bsr_stale = false;
+  return ONSAS_OK;
+}
+
 int launch_x_to_double(Ctx* c, double* out_dev) {
   const int64_t n = 3 * c->nn;
   if (c->precision == ONSAS_PRECISION_FP64)
@@ -872,6 +917,7 @@ int launch_x_to_double(Ctx* c, double* out_dev) {
 }
 
 int launch_convert_K(Ctx* c, double* out_dev) {
+  ONSAS_TRY(ensure_bsr(c));
   const int64_t n = 9 * c->nnzb;
   if (c->precision == ONSAS_PRECISION_FP64)
     hipLaunchKernelGGL(k_convert<double>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,

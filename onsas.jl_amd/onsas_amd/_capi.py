@@ -59,7 +59,8 @@ class Timings(C.Structure):
     _fields_ = [("assemble_ms", C.c_double), ("scatter_ms", C.c_double), ("solve_ms", C.c_double),
                 ("pcg_ms", C.c_double), ("n_assemble", C.c_int64), ("n_steps", C.c_int64),
                 ("pcg_iters", C.c_int64), ("gmres_iters", C.c_int64),
-                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int64)]
+                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int64),
+                ("elem_ms", C.c_double), ("assembly_mode", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -7,6 +7,9 @@ its row block; converged displacements to 1e-8 relative of max|U| when the
 Newton residual is driven below 1e-10 (both precisions iterate to the same FP64
 residual, so the converged states must agree).
 """
+import contextlib
+import os
+
 import numpy as np
 import pytest
 
@@ -22,8 +25,28 @@ MATS = {"svk": (_capi.MAT_SVK, (0.6e6, 0.4e6)), "nh": (_capi.MAT_NEOHOOKEAN, (1.
         "linear": (_capi.MAT_LINEAR, (1.0e6, 0.3))}
 
 
+@contextlib.contextmanager
+def _assembly_mode(mode):
+    """ONSAS_ASSEMBLY is read when a context is finalized."""
+    old = os.environ.get("ONSAS_ASSEMBLY")
+    if mode is not None:
+        os.environ["ONSAS_ASSEMBLY"] = mode
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop("ONSAS_ASSEMBLY", None)
+        else:
+            os.environ["ONSAS_ASSEMBLY"] = old
+
+
 def _ctx(prec, coords, tets=None, mat="svk", trusses=None, strain=0, E=1.0, A=None, fixed=(),
-         faces=None, G=None):
+         faces=None, G=None, assembly=None):
+    with _assembly_mode(assembly):
+        return _ctx_build(prec, coords, tets, mat, trusses, strain, E, A, fixed, faces, G)
+
+
+def _ctx_build(prec, coords, tets, mat, trusses, strain, E, A, fixed, faces, G):
     c = _capi.Context(PREC[prec])
     c.set_nodes(coords)
     if tets is not None:
@@ -105,17 +128,20 @@ def _bsr_vs_oracle(Kg, Ko, fixed, n, tol):
     assert np.abs(Kg[np.ix_(free, fixed)]).max(initial=0) == 0
 
 
+@pytest.mark.parametrize("assembly", ["gather", "scatter"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("mat", ["svk", "nh", "linear"])
-def test_assembly_matches_oracle(oracle, prec, mat):
-    """Coloured block-CSR assembly of f_int and K vs the oracle's COO->CSR assembly."""
-    print(f"[gpu] assembly {mat} {prec}", flush=True)
+def test_assembly_matches_oracle(oracle, prec, mat, assembly):
+    """Gather (per-block contribution lists) and coloured-scatter assembly of
+    f_int and K vs the oracle's COO->CSR assembly."""
+    print(f"[gpu] assembly {mat} {prec} {assembly}", flush=True)
     s, sm = workloads.cube_extension(n=4, jitter=0.1, seed=3)
     fixed = s.fixed_dofs
     rng = np.random.default_rng(5)
     U = rng.normal(scale=0.02, size=3 * sm.nn)
     U[fixed] = 0
-    c = _ctx(prec, sm.coords, sm.tets, mat, fixed=fixed)
+    c = _ctx(prec, sm.coords, sm.tets, mat, fixed=fixed, assembly=assembly)
+    assert c.timings()["assembly_mode"] == 0  # no assembly yet
     c.set_U(U)
     fint = c.assemble(want_fint=True)
     Kg = c.tangent_bsr()
@@ -130,16 +156,63 @@ def test_assembly_matches_oracle(oracle, prec, mat):
     yo = out["K"] @ xf
     yo[fixed] = x[fixed]
     np.testing.assert_allclose(y, yo, rtol=0, atol=(1e-11 if prec == "fp64" else 3e-6) * np.abs(yo).max())
+    assert c.timings()["assembly_mode"] == (1 if assembly == "gather" else 0)
     c.close()
 
 
-def test_assembly_is_deterministic():
+@pytest.mark.parametrize("assembly", ["gather", "scatter"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_assembly_mixed_groups_vs_oracle(oracle, prec, assembly):
+    """Tets and trusses in one model (two element groups sharing nodes), plus a
+    fully fixed node that no element touches (identity block-Jacobi row)."""
+    print(f"[gpu] assembly tets+trusses {prec} {assembly}", flush=True)
+    sm = meshes.cube(3, jitter=0.1, seed=4)
+    X = np.vstack([sm.coords, [[2.0, 2.0, 2.0]]])  # the isolated node
+    nn = len(X)
+    bars = np.array([[0, 5], [3, 17], [10, 40], [21, 63], [1, 2]], dtype=np.int32)
+    A = np.array([1e-3, 2e-3, 1.5e-3, 1e-3, 3e-3])
+    fixed = np.concatenate([np.arange(3 * 0, 3 * 0 + 3), [3 * 4 + 1], np.arange(3 * (nn - 1), 3 * nn)])
+    rng = np.random.default_rng(8)
+    U = rng.normal(scale=0.02, size=3 * nn)
+    U[fixed] = 0
+    with _assembly_mode(assembly):
+        c = _capi.Context(PREC[prec])
+        c.set_nodes(X)
+        c.add_tet_group(sm.tets, _capi.MAT_SVK, MATS["svk"][1], None)
+        c.add_truss_group(bars, 1, 2e7, A)
+        c.set_fixed_dofs(fixed)
+        c.finalize()
+    c.set_U(U)
+    fint = c.assemble(want_fint=True)
+    Kg = c.tangent_bsr()
+    m = oracle.Model(X, tets=sm.tets, tet_mat=oracle.MAT_SVK, tet_p=MATS["svk"][1], trusses=bars,
+                     truss_strain=oracle.STRAIN_GREEN, truss_E=2e7, truss_A=A, fixed_dofs=fixed)
+    out = m.assemble(U)
+    np.testing.assert_allclose(fint, out["fint"], rtol=0, atol=1e-12 * np.abs(out["fint"]).max())
+    # the isolated node has no block in the pattern: compare the rest
+    m3 = 3 * (nn - 1)
+    assert np.abs(Kg.toarray()[m3:]).max() == 0
+    _bsr_vs_oracle(Kg.tocsr()[:m3, :m3], out["K"].tocsr()[:m3, :m3], fixed[fixed < m3], m3,
+                   1e-12 if prec == "fp64" else 2e-6)
+    # a solve through the block-Jacobi PCG touches the identity row of the isolated node
+    b = rng.normal(size=3 * nn)
+    x, it, rr = c.linear_solve(b)
+    free = np.setdiff1d(np.arange(3 * nn), fixed)
+    Kff = Kg.tocsr()[free][:, free]  # the engine's own (FP32 in mixed mode) tangent
+    res = Kff @ x[free] - b[free]
+    # FP32 PCG: the true residual can trail the recurrence by ~ eps32 cond(K) (cond ~ 7e3 here)
+    assert np.linalg.norm(res) <= (1e-7 if prec == "fp64" else 5e-4) * np.linalg.norm(b[free])
+    c.close()
+
+
+@pytest.mark.parametrize("assembly", ["gather", "scatter"])
+def test_assembly_is_deterministic(assembly):
     """Two assemblies of the same state give the same bits (no float atomics)."""
-    print("[gpu] assembly determinism", flush=True)
+    print(f"[gpu] assembly determinism {assembly}", flush=True)
     sm = meshes.cube(8, jitter=0.1, seed=9)
     U = np.random.default_rng(2).normal(scale=0.01, size=3 * sm.nn)
     for prec in ("mixed", "fp64"):
-        c = _ctx(prec, sm.coords, sm.tets, "nh")
+        c = _ctx(prec, sm.coords, sm.tets, "nh", assembly=assembly)
         c.set_U(U)
         f1 = c.assemble(True)
         K1 = c.tangent_bsr().data.copy()
