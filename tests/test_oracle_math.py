@@ -259,3 +259,28 @@ def test_newton_criteria_semantics(oracle):
     assert rec[0]["criterion"] == 1 and rec[0]["r_rel"] <= 1e-10
     _, _, rec = m.newton(F, rel_U_tol=1e-14, rel_res_tol=1e-10, max_iter=0)
     assert rec[0]["criterion"] == 3 and rec[0]["iters"] == 1
+
+
+def test_cg_restatement(oracle):
+    """IterativeSolvers cg! as the reference calls it (x0 = 0, unpreconditioned,
+    stop at ||r|| <= max(reltol ||b||, abstol) or maxiter): exact arithmetic
+    facts fix it -- k distinct eigenvalues => convergence in k iterations, the
+    solution of an SPD system, the maxiter cap and b = 0 -> x = 0 in 0 steps."""
+    import scipy.sparse as sp
+    # diagonal with 5 distinct eigenvalues: CG terminates after 5 iterations
+    d = np.repeat([1.0, 2.0, 5.0, 11.0, 30.0], 8)
+    b = RNG.normal(size=d.size)
+    x, it = oracle.cg(sp.diags(d).tocsr(), b, reltol=1e-12)
+    assert it == 5
+    np.testing.assert_allclose(x, b / d, rtol=1e-10)
+    # SPD system vs a direct solve
+    A = sp.random(60, 60, density=0.1, random_state=3)
+    A = (A @ A.T + sp.identity(60) * 2.0).tocsr()
+    b = RNG.normal(size=60)
+    x, it = oracle.cg(A, b, reltol=1e-12)
+    np.testing.assert_allclose(A @ x, b, rtol=0, atol=1e-10 * np.abs(b).max())
+    assert 0 < it <= 60
+    x3, it3 = oracle.cg(A, b, reltol=1e-12, maxiter=3)
+    assert it3 == 3 and np.linalg.norm(A @ x3 - b) > 1e-6 * np.linalg.norm(b)
+    x0, it0 = oracle.cg(A, np.zeros(60))
+    assert it0 == 0 and not x0.any()
