@@ -208,6 +208,14 @@ def main():
 
     head = runs["mixed+forcing"]
     h_spmv = spmv_roofline(head, 4)
+    # HBM traffic per k_pcg_A launch from the committed rocprof summary of this
+    # workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-kernel calibration)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "newton_cube64_kernels.md")
+    if args.n == 64 and os.path.exists(prof):
+        import re
+        mt = re.search(r"traffic_bytes_per_launch k_pcg_A<float>: (\d+)", open(prof).read())
+        traffic = int(mt.group(1)) if mt else None
     result = {
         "metric": "Newton solve wall time & assembly Melem/s, mixed vs FP64 on 1 GPU at resid<=1e-8",
         "value": round(head["wall_s"], 4),
@@ -229,7 +237,7 @@ def main():
                    "rel_res_force_tol": 1e-8},
         "roofline": {"bound": "hbm", "achieved": round(h_spmv["achieved_GBps"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(h_spmv["achieved_GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(h_spmv["achieved_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_pcg_A (sliced-ELL 3x3 SpMV + p update, FP32)",
                      "bytes_per_launch": h_spmv["bytes_per_launch"],
                      "avg_launch_ms": round(h_spmv["avg_launch_ms"], 5)},

@@ -23,6 +23,8 @@
 #include <cstdlib>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "engine.hpp"
 
 namespace onsas {
@@ -649,14 +651,19 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
   KT* P[2] = {(KT*)c->d_pp0, (KT*)c->d_pp1};
   double* RZ[2] = {part(c, kPartRZ0), part(c, kPartRZ1)};
   PcgState* st = &c->d_sc->pcg;
-  if (ea) (void)hipEventRecord(ea, s);
   const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
-  hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
-                     c->d_slice_rounds, rstride, c->d_sell_col,
-                     (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
-                     RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, c->chunk,
-                     c->nn);
-  if (eb) (void)hipEventRecord(eb, s);
+  if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
+    hipExtLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, ea, eb, 0,
+                          c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
+                          (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par],
+                          (KT*)c->d_pq, RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ),
+                          st, c->np, c->chunk, c->nn);
+  else
+    hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
+                       c->d_slice_rounds, rstride, c->d_sell_col,
+                       (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
+                       RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np,
+                       c->chunk, c->nn);
   hipLaunchKernelGGL(k_pcg_B<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                      (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz, RZ[par],
                      part(c, kPartPQ), RZ[1 - par], part(c, kPartRR), st, c->np, c->chunk, c->nn);
