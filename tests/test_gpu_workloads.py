@@ -109,3 +109,41 @@ def test_cylinder_lame_on_gpu():
                         np.abs(ur[sm.node_sets["outer"]] / ex_o - 1).max()))
     print("  Lame errors", errs, flush=True)
     assert errs[1] < 0.6 * errs[0] and errs[2] < 0.6 * errs[1] and errs[2] < 0.02
+
+
+def test_gmsh_mesh_solves_like_the_direct_mesh(tmp_path):
+    """A cube written as an MSH 2.2 file with the reference's physical-name
+    labels, read back, solved on the GPU: the same displacements as the cube
+    built directly (same arrays, so the same bits), written to ParaView files."""
+    from onsas_amd import meshes
+    from onsas_amd.boundary_conditions import FixedField, Pressure, StructuralBoundaryCondition
+    from onsas_amd.entities import Tetrahedra, TriangularFaces
+    from onsas_amd.gmsh import MshFile, mesh_from_msh, write_msh
+    from onsas_amd.materials import SVK
+    from onsas_amd.structure import StructuralMaterial, Structure
+    from onsas_amd.vtk import read_vtu, write_vtk
+    print("[gpu] gmsh cube", flush=True)
+    sm = meshes.cube(3, jitter=0.1, seed=3)
+    fn = write_msh(str(tmp_path / "cube.msh"), sm.coords,
+                   [("svk_tetrahedron_", 4, sm.tets), ("_triangle_tension", 2, sm.faces["x1"]),
+                    ("_triangle_fixed-ux", 2, sm.faces["x0"]),
+                    ("_triangle_fixed-uy", 2, sm.faces["y0"]),
+                    ("_triangle_fixed-uz", 2, sm.faces["z0"])])
+    gm = mesh_from_msh(MshFile(fn), {"tetrahedron": Tetrahedra, "triangle": TriangularFaces})
+    bcs = {FixedField("u", [1], "ux"): [gm.face_set("fixed-ux")],
+           FixedField("u", [2], "uy"): [gm.face_set("fixed-uy")],
+           FixedField("u", [3], "uz"): [gm.face_set("fixed-uz")],
+           Pressure("u", lambda t: -3e5 * t, "tension"): [gm.face_set("tension")]}
+    s = Structure(gm, StructuralMaterial({SVK(E=1e6, nu=0.3, label="svk"): gm.element_set("tetrahedron")}),
+                  StructuralBoundaryCondition(bcs))
+    alg = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-14, rel_res_force_tol=1e-10, max_iter=30))
+    sol = solve(NonLinearStaticAnalysis(s, NSTEPS=3), alg, MI355XBackend(precision="mixed"))
+    s_ref, _ = workloads.cube_extension(n=3, jitter=0.1, seed=3)
+    ref = solve(NonLinearStaticAnalysis(s_ref, NSTEPS=3), alg, MI355XBackend(precision="mixed"))
+    for k in range(3):
+        np.testing.assert_array_equal(sol.U[k], ref.U[k])
+    pvd = write_vtk(sol, str(tmp_path / "cube"))
+    d = read_vtu(str(tmp_path / "cube_timestep_3.vtu"))
+    np.testing.assert_array_equal(d["point"]["Displacement"], sol.U[2].reshape(-1, 3))
+    np.testing.assert_array_equal(d["cell"]["σxx"], sol.stress[2][0][:, 0, 0])
+    assert pvd.endswith("cube.pvd")
