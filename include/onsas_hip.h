@@ -241,6 +241,17 @@ int onsas_adjoint_shear(onsas_ctx* ctx, int32_t group, const double* dJdU, doubl
  * onsas_assemble uses the new moduli. */
 int onsas_set_shear_moduli(onsas_ctx* ctx, int32_t group, const double* G);
 
+/* Point location in a tet group, reference configuration (replaces
+ * PointEvalHandler(mesh, points) with the Serial search of
+ * src/Meshes/Handlers.jl:67-87 and src/Meshes/Searches.jl:64-82, and the
+ * weights of src/Entities/Tetrahedrons.jl:311-313): for each point, the first
+ * element in user order that contains it (barycentric coordinates >= -1e-12)
+ * in elem[i] (-1 outside the group), its four barycentric weights in
+ * weights[4i..4i+3] (0 outside).  Interpolating a nodal field at the point is
+ * sum_a weights[a] * field(node a).  A bin grid is built on the first call. */
+int onsas_locate_points(onsas_ctx* ctx, int32_t group, int64_t npts, const double* x,
+                        const double* y, const double* z, int64_t* elem, double* weights);
+
 /* Timers. */
 int onsas_get_timings(onsas_ctx* ctx, onsas_timings* t);
 int onsas_reset_timings(onsas_ctx* ctx);

@@ -2,6 +2,7 @@
 // include/onsas_hip.h).  Argument checking, ownership and the orchestration of
 // one Newton iteration; the numerics live in element_kernels.hip and
 // solver_kernels.hip, the setup in setup.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -224,7 +225,8 @@ void onsas_destroy(onsas_ctx* ctx) {
   if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
   for (auto& g : c->groups)
     for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm,
-                    g.d_state, (void*)g.d_fe, (void*)g.d_cidx})
+                    g.d_state, (void*)g.d_fe, (void*)g.d_cidx, (void*)g.locate.d_ptr,
+                    (void*)g.locate.d_elem})
       if (p) (void)hipFree(p);
   void* ptrs[] = {c->d_x, c->d_y, c->d_z, c->d_fixmask, c->d_row_ptr, c->d_col, c->d_diag, c->d_K,
                   c->d_Dinv, c->d_U, c->d_Uasm, c->d_fint, c->d_fext, c->d_res, c->d_nodal, c->d_px, c->d_pr,
@@ -628,6 +630,104 @@ int onsas_set_shear_moduli(onsas_ctx* ctx, int32_t group, const double* G) {
   g.per_elem_G = true;
   g.Gel.assign(G, G + g.ne);
   c->assembled = false;  // K and f_int belong to the old moduli
+  return ONSAS_OK;
+}
+
+namespace {
+// Uniform bin grid of a tet group (reference configuration): about two
+// elements per bin; each bin lists, in user element order, the elements whose
+// bounding box (widened by a relative 1e-9) overlaps it.
+int build_locate_grid(Ctx* c, ElemGroup& g) {
+  onsas::LocateGrid& lg = g.locate;
+  const int64_t ne = g.ne;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  const std::vector<double>* P[3] = {&c->X, &c->Y, &c->Z};
+  for (int64_t k = 0; k < 4 * ne; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const double v = (*P[d])[g.conn[k]];
+      lo[d] = std::min(lo[d], v);
+      hi[d] = std::max(hi[d], v);
+    }
+  double ext[3], vol = 1;
+  const double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  const double pad = 1e-9 * diag;
+  for (int d = 0; d < 3; ++d) {
+    lo[d] -= pad;
+    hi[d] += pad;
+    ext[d] = hi[d] - lo[d];
+    vol *= ext[d];
+  }
+  const double h = std::cbrt(vol / std::max<double>(1.0, ne / 2.0));
+  int64_t nbin = 1;
+  for (int d = 0; d < 3; ++d) {
+    lg.nb[d] = (int)std::min<double>(512.0, std::max(1.0, std::ceil(ext[d] / h)));
+    lg.lo[d] = lo[d];
+    lg.inv_h[d] = lg.nb[d] / ext[d];
+    nbin *= lg.nb[d];
+  }
+  std::vector<int32_t> pos(ne);
+  for (int64_t k = 0; k < ne; ++k) pos[g.perm[k]] = (int32_t)k;
+  auto bin_range = [&](int64_t u, int b0[3], int b1[3]) {
+    for (int d = 0; d < 3; ++d) {
+      double mn = 1e300, mx = -1e300;
+      for (int a = 0; a < 4; ++a) {
+        const double v = (*P[d])[g.conn[4 * u + a]];
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+      }
+      b0[d] = std::max(0, (int)std::floor((mn - pad - lg.lo[d]) * lg.inv_h[d]));
+      b1[d] = std::min(lg.nb[d] - 1, (int)std::floor((mx + pad - lg.lo[d]) * lg.inv_h[d]));
+    }
+  };
+  std::vector<int64_t> ptr(nbin + 1, 0);
+  int b0[3], b1[3];
+  for (int64_t u = 0; u < ne; ++u) {
+    bin_range(u, b0, b1);
+    for (int z = b0[2]; z <= b1[2]; ++z)
+      for (int y = b0[1]; y <= b1[1]; ++y)
+        for (int x = b0[0]; x <= b1[0]; ++x) ptr[((int64_t)z * lg.nb[1] + y) * lg.nb[0] + x + 1]++;
+  }
+  for (int64_t b = 0; b < nbin; ++b) ptr[b + 1] += ptr[b];
+  std::vector<int32_t> list(std::max<int64_t>(ptr[nbin], 1));
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t u = 0; u < ne; ++u) {  // user order: every bin's list is sorted by it
+    bin_range(u, b0, b1);
+    for (int z = b0[2]; z <= b1[2]; ++z)
+      for (int y = b0[1]; y <= b1[1]; ++y)
+        for (int x = b0[0]; x <= b1[0]; ++x) list[fill[((int64_t)z * lg.nb[1] + y) * lg.nb[0] + x]++] = pos[u];
+  }
+  ONSAS_HIP_TRY(c, hipMalloc((void**)&lg.d_ptr, ptr.size() * sizeof(int64_t)));
+  ONSAS_HIP_TRY(c, hipMemcpy(lg.d_ptr, ptr.data(), ptr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  ONSAS_HIP_TRY(c, hipMalloc((void**)&lg.d_elem, list.size() * sizeof(int32_t)));
+  ONSAS_HIP_TRY(c, hipMemcpy(lg.d_elem, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  lg.built = true;
+  return ONSAS_OK;
+}
+}  // namespace
+
+int onsas_locate_points(onsas_ctx* ctx, int32_t group, int64_t npts, const double* x,
+                        const double* y, const double* z, int64_t* elem, double* weights) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (group < 0 || group >= (int32_t)c->groups.size()) return set_err(c, ONSAS_ERR_ARG, "bad group");
+  ElemGroup& g = c->groups[group];
+  if (g.type != onsas::kTet) return set_err(c, ONSAS_ERR_UNSUPPORTED, "point location needs a tet group");
+  if (npts < 0 || (npts > 0 && (!x || !y || !z || !elem || !weights)))
+    return set_err(c, ONSAS_ERR_ARG, "onsas_locate_points: bad arguments");
+  if (npts == 0) return ONSAS_OK;
+  if (!g.locate.built) ONSAS_TRY(build_locate_grid(c, g));
+  DevScratch dx, dy, dz, dw, de;
+  ONSAS_TRY(to_device(c, dx, x, npts));
+  ONSAS_TRY(to_device(c, dy, y, npts));
+  ONSAS_TRY(to_device(c, dz, z, npts));
+  ONSAS_TRY(to_device(c, dw, nullptr, 4 * (size_t)npts));
+  ONSAS_TRY(to_device(c, de, nullptr, npts));  // int64 results in a double-sized buffer
+  ONSAS_TRY(onsas::launch_locate(c, group, dx.p, dy.p, dz.p, npts, 1e-12, (int64_t*)de.p, dw.p));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(elem, de.p, npts * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(weights, dw.p, 4 * npts * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return ONSAS_OK;
 }
 

@@ -652,6 +652,52 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs ga, const SellGroup* 
   }
 }
 
+// Point location (PointEvalHandler, src/Meshes/Handlers.jl:67-87): one lane per
+// query point, the candidates of its grid bin scanned in user element order;
+// the first tetrahedron whose barycentric coordinates are all >= -tol holds the
+// point (the reference's Serial search keeps the first containing element,
+// src/Meshes/Searches.jl:64-82).  lambda_a = dN_a . (p - X_b) for any node b != a
+// (linear shape functions vanish at the other nodes), i.e. the weights of
+// Tetrahedrons.jl:311-313.
+__global__ void k_locate(TetArgs ta, const double* __restrict__ X, const double* __restrict__ Y,
+                         const double* __restrict__ Z, const double* __restrict__ px,
+                         const double* __restrict__ py, const double* __restrict__ pz, int64_t np,
+                         const int64_t* __restrict__ bin_ptr, const int32_t* __restrict__ bin_elem,
+                         double3 lo, double3 inv_h, int3 nb, const int32_t* __restrict__ perm,
+                         double tol, int64_t* __restrict__ out_elem, double* __restrict__ out_w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double p[3] = {px[i], py[i], pz[i]};
+  int64_t found = -1;
+  double w[4] = {0, 0, 0, 0};
+  const int bx = (int)floor((p[0] - lo.x) * inv_h.x), by = (int)floor((p[1] - lo.y) * inv_h.y),
+            bz = (int)floor((p[2] - lo.z) * inv_h.z);
+  if (bx >= 0 && by >= 0 && bz >= 0 && bx < nb.x && by < nb.y && bz < nb.z) {
+    const int64_t b = ((int64_t)bz * nb.y + by) * nb.x + bx;
+    for (int64_t q = bin_ptr[b]; q < bin_ptr[b + 1] && found < 0; ++q) {
+      const int64_t e = bin_elem[q];
+      double lam[4];
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int32_t nbn = ta.conn[((a + 1) & 3) * ta.ne + e];
+        const double d0 = p[0] - X[nbn], d1 = p[1] - Y[nbn], d2 = p[2] - Z[nbn];
+        lam[a] = ta.geo[(a * 3) * ta.ne + e] * d0 + ta.geo[(a * 3 + 1) * ta.ne + e] * d1 +
+                 ta.geo[(a * 3 + 2) * ta.ne + e] * d2;
+        inside = inside && lam[a] >= -tol;
+      }
+      if (inside) {
+        found = perm[e];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) w[a] = lam[a];
+      }
+    }
+  }
+  out_elem[i] = found;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) out_w[4 * i + a] = w[a];
+}
+
 // Block-Jacobi rows of nodes without a diagonal block (fully fixed, no element).
 template <typename KT>
 __global__ void k_dinv_identity(const int32_t* __restrict__ diag, KT* __restrict__ Dinv, int64_t nn) {
@@ -1102,6 +1148,19 @@ int launch_tangent_apply_f64(Ctx* c, const double* v, double* y) {
   }
   hipLaunchKernelGGL(k_fixed_copy, dim3(grid_for(c->nn, 256)), dim3(256), 0, s, v, y, c->d_fixmask,
                      c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+int launch_locate(Ctx* c, int gi, const double* px, const double* py, const double* pz,
+                  int64_t np, double tol, int64_t* elem, double* w) {
+  const ElemGroup& g = c->groups[gi];
+  const LocateGrid& lg = g.locate;
+  hipLaunchKernelGGL(k_locate, dim3(grid_for(np, 256)), dim3(256), 0, c->stream,
+                     tet_args(g, 0, g.ne), c->d_x, c->d_y, c->d_z, px, py, pz, np, lg.d_ptr,
+                     lg.d_elem, make_double3(lg.lo[0], lg.lo[1], lg.lo[2]),
+                     make_double3(lg.inv_h[0], lg.inv_h[1], lg.inv_h[2]),
+                     make_int3(lg.nb[0], lg.nb[1], lg.nb[2]), g.d_perm, tol, elem, w);
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
 }

@@ -45,6 +45,16 @@ constexpr int kMaxPartials = 1024; // upper bound of the reduction grid (multipl
 
 enum GroupType { kTet = 0, kTruss = 1 };
 
+// Uniform grid over a tet group's bounding box for point location: bin b
+// lists (in user element order) the elements whose bounding box overlaps it.
+struct LocateGrid {
+  bool built = false;
+  double lo[3] = {0, 0, 0}, inv_h[3] = {0, 0, 0};
+  int nb[3] = {0, 0, 0};
+  int64_t* d_ptr = nullptr;
+  int32_t* d_elem = nullptr;  // array positions
+};
+
 struct ElemGroup {
   int type = kTet;
   int mat = 0;          // ONSAS_MAT_* for tets, strain model for trusses
@@ -75,6 +85,7 @@ struct ElemGroup {
   int nstate = 0;
   bool hasM = false;     // SVK: the c2 s_ab M term needs M = F F' (M = I otherwise)
   int64_t goff = 0;      // first global element id of the group (contribution lists)
+  LocateGrid locate;     // built on the first point location
 };
 
 // Tet state planes (stride ne): 0 V c1, 1 V c2, 2..13 v[a][i], 14..23 g_ab (the
@@ -235,6 +246,8 @@ int launch_adjoint_shear(Ctx* c, int g, const double* lambda_dev, double* dJdG_d
 int launch_gather_init(Ctx* c);                                 // zero padding, identity Dinv
 int ensure_bsr(Ctx* c);                                         // d_K from d_Ksell if stale
 int launch_tangent_apply_f64(Ctx* c, const double* v_dev, double* y_dev);  // y = K(U) v, FP64
+int launch_locate(Ctx* c, int g, const double* px, const double* py, const double* pz, int64_t np,
+                  double tol, int64_t* elem, double* w);
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2

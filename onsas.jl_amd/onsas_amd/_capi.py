@@ -32,7 +32,7 @@ EXPORTED = (
     "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_set_U", "onsas_get_fext",
     "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
     "onsas_tangent_matvec", "onsas_element_forces", "onsas_get_stress_strain",
-    "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_get_timings", "onsas_reset_timings",
+    "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points", "onsas_get_timings", "onsas_reset_timings",
     "onsas_inject_error",
 )
 
@@ -116,6 +116,7 @@ def load():
         "onsas_linear_solve": ([_vp, _vp, _vp, _i32p, _dp], C.c_int),
         "onsas_adjoint_shear": ([_vp, C.c_int32, _vp, _vp, _i32p], C.c_int),
         "onsas_set_shear_moduli": ([_vp, C.c_int32, _vp], C.c_int),
+        "onsas_locate_points": ([_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, _vp], C.c_int),
         "onsas_get_timings": ([_vp, C.POINTER(Timings)], C.c_int),
         "onsas_reset_timings": ([_vp], C.c_int),
         "onsas_inject_error": ([_vp, C.c_int32], C.c_int),
@@ -330,6 +331,16 @@ class Context:
     def set_shear_moduli(self, group, G):
         g = _f64(G, self.group_sizes[group])
         self._check(self.L.onsas_set_shear_moduli(self.h, group, g.ctypes.data))
+
+    def locate_points(self, group, points):
+        P = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        n = P.shape[0]
+        x, y, z = (np.ascontiguousarray(P[:, d]) for d in range(3))
+        elem = np.empty(n, dtype=np.int64)
+        w = np.empty(4 * n)
+        self._check(self.L.onsas_locate_points(self.h, group, n, x.ctypes.data, y.ctypes.data,
+                                               z.ctypes.data, elem.ctypes.data, w.ctypes.data))
+        return elem, w.reshape(n, 4)
 
     def timings(self):
         t = Timings()

@@ -147,3 +147,46 @@ def test_gmsh_mesh_solves_like_the_direct_mesh(tmp_path):
     np.testing.assert_array_equal(d["point"]["Displacement"], sol.U[2].reshape(-1, 3))
     np.testing.assert_array_equal(d["cell"]["σxx"], sol.stress[2][0][:, 0, 0])
     assert pvd.endswith("cube.pvd")
+
+
+def _brute_locate(coords, tets, P, tol=1e-12):
+    """First element (in element order) whose barycentric coordinates
+    lambda = A^-T [1; p], A rows [1 x y z] (Tetrahedrons.jl:287-313), are all
+    >= -tol; -1 outside."""
+    out = np.full(len(P), -1)
+    W = np.zeros((len(P), 4))
+    for e, t in enumerate(tets):
+        A = np.hstack([np.ones((4, 1)), coords[t]])
+        lam = np.linalg.solve(A.T, np.vstack([np.ones(len(P)), P.T])).T  # (np, 4)
+        hit = (out < 0) & np.all(lam >= -tol, axis=1)
+        out[hit] = e
+        W[hit] = lam[hit]
+    return out, W
+
+
+def test_point_location_matches_brute_force():
+    """GPU bin-grid search vs a brute-force search over every element, with
+    points inside, outside, on nodes and on shared faces (the first element in
+    element order wins, as the reference's Serial search)."""
+    from onsas_amd.backend import GpuStaticState
+    from onsas_amd.points import PointEvalHandler, interpolate
+    print("[gpu] point location", flush=True)
+    s, sm = workloads.cube_extension(n=4, jitter=0.15, seed=11)
+    st = GpuStaticState(s, MI355XBackend(precision="fp64"))
+    rng = np.random.default_rng(3)
+    P = np.vstack([rng.uniform(-0.1, 1.1, size=(400, 3)),          # some outside
+                   sm.coords[rng.choice(sm.nn, 40, replace=False)],  # on nodes
+                   sm.coords[sm.tets[:30]].mean(axis=1),               # centroids
+                   sm.coords[sm.tets[:30, :3]].mean(axis=1)])          # on faces
+    h = PointEvalHandler(st, P)
+    eb, wb = _brute_locate(sm.coords, sm.tets, P)
+    elem = np.full(len(P), -1)
+    elem[h.in_mesh_points_idx] = h.in_mesh_elements_idx
+    np.testing.assert_array_equal(elem, eb)
+    np.testing.assert_allclose(h.weights, wb[h.in_mesh_points_idx], rtol=0, atol=1e-12)
+    assert len(h.not_in_mesh_points_idx) > 0 and len(h.in_mesh_points_idx) > 300
+    # linear fields are interpolated exactly
+    f = sm.coords @ np.array([1.0, -2.0, 0.5]) + 3.0
+    np.testing.assert_allclose(interpolate(h, f), h.in_mesh_points @ np.array([1.0, -2.0, 0.5]) + 3.0,
+                               rtol=0, atol=1e-12)
+    st.close()
