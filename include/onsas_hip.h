@@ -51,6 +51,9 @@ typedef struct onsas_ctx onsas_ctx;
 #define ONSAS_LINSOLVE_PCG 1   /* PCG only (block-Jacobi)                             */
 #define ONSAS_LINSOLVE_GMRES 2 /* restarted GMRES only (block-Jacobi, right precond.) */
 
+#define ONSAS_PRECOND_BLOCK_JACOBI 0 /* 3x3 nodal block-Jacobi                          */
+#define ONSAS_PRECOND_AMG 1          /* aggregation multigrid V-cycle (rigid-body modes) */
+
 /* Linear-solver settings; replace the tolerances of
  * src/StructuralSolvers/StructuralSolvers.jl:229-234 (abstol 0, reltol sqrt(eps),
  * maxiter = number of free dofs) and the LinearSolve algorithm argument of
@@ -68,6 +71,7 @@ typedef struct {
     double eta_max;        /* forcing: loosest inner tolerance (<= 0: 0.1)               */
     double eta_first;      /* forcing: inner tolerance of the first Newton iteration of a
                               load step (<= 0: 1e-3)                                      */
+    int32_t precond;       /* PCG preconditioner: ONSAS_PRECOND_* (GMRES: block-Jacobi)  */
 } onsas_linear_settings;
 
 /* Per-Newton-iteration record: the four norms of

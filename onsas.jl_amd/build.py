@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "onsas_amd", "libonsas_hip.so")
-SOURCES = ["setup.hip", "element_kernels.hip", "solver_kernels.hip", "capi.hip"]
+SOURCES = ["setup.hip", "element_kernels.hip", "solver_kernels.hip", "amg.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -173,6 +173,7 @@ int assemble_now(Ctx* c) {
   ONSAS_TRY(onsas::launch_assemble(c));
   if (!c->gather) ONSAS_TRY(onsas::launch_pack_sell(c));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  c->amg.stale = true;  // its Galerkin products belong to the previous tangent
   return ONSAS_OK;
 }
 
@@ -223,6 +224,7 @@ void onsas_destroy(onsas_ctx* ctx) {
   Ctx* c = &ctx->impl;
   (void)hipStreamSynchronize(c->stream);
   if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
+  onsas::amg_free(c);
   for (auto& g : c->groups)
     for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm,
                     g.d_state, (void*)g.d_fe, (void*)g.d_cidx, (void*)g.locate.d_ptr,
@@ -351,6 +353,8 @@ int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s) {
   if (!c || !s) return ONSAS_ERR_ARG;
   if (s->kind < ONSAS_LINSOLVE_AUTO || s->kind > ONSAS_LINSOLVE_GMRES)
     return set_err(c, ONSAS_ERR_ARG, "unknown linear solver kind");
+  if (s->precond != ONSAS_PRECOND_BLOCK_JACOBI && s->precond != ONSAS_PRECOND_AMG)
+    return set_err(c, ONSAS_ERR_ARG, "unknown preconditioner");
   c->lin = *s;
   if (c->lin.gmres_restart <= 0) c->lin.gmres_restart = 40;
   if (c->lin.check_every <= 0) c->lin.check_every = 16;

@@ -105,6 +105,39 @@ struct SellGroup {
   int64_t cbase;   // first contribution entry (entries of round q at cbase + 64 q + lane)
 };
 
+// Aggregation multigrid (amg.hip).  Level 0 is the fine tangent (its values
+// are the sliced-ELL copy and the block-Jacobi inverse of the context); levels
+// >= 1 have 6 dofs per node and 6x6 block-CSR values.
+struct AmgLevel {
+  int bs = 6;                         // dofs per node
+  int64_t n = 0;                      // nodes
+  std::vector<int32_t> row_ptr, col, diag;  // pattern (levels >= 1)
+  int64_t nnzb = 0;
+  int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_diag = nullptr;
+  void* d_A = nullptr;                // nnzb x 36 KT
+  void* d_Dinv = nullptr;             // n x 36 KT
+  double omega = 1.0;                 // smoother damping, 1 / (1.1 lambda_max(D^-1 A))
+  void *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;  // cycle vectors (KT, bs n)
+  // transfer to the next coarser level
+  int32_t* d_agg = nullptr;           // node -> coarse node
+  void* d_P = nullptr;                // bs x 6 per node (KT)
+  int64_t* d_aptr = nullptr;          // coarse node -> member nodes (CSR)
+  int32_t* d_anodes = nullptr;
+  // Galerkin lists of this level's blocks (levels >= 1)
+  int64_t* d_gptr = nullptr;          // nnzb + 1
+  int64_t* d_goff = nullptr;          // value offset of the finer block
+  int32_t *d_gi = nullptr, *d_gj = nullptr;  // its finer row and column nodes
+};
+
+struct Amg {
+  bool built = false, stale = true, omega_set = false;
+  std::vector<AmgLevel> lev;
+  int nc = 0;                         // dofs of the coarsest level (dense inverse)
+  double* d_dense = nullptr;          // nc x nc FP64 work
+  void* d_Ainv = nullptr;             // nc x nc KT
+  double* d_part = nullptr;           // power-iteration partial sums
+};
+
 struct PcgState {     // device-resident scalars of the PCG iteration
   double tol2;        // (rtol ||b||)^2
   double rr_last;     // ||r_k||^2 at the last convergence check
@@ -209,9 +242,13 @@ struct Ctx {
   // linear solver settings
   onsas_linear_settings lin{};
 
-  // PCG graph
+  // PCG graph (a chunk of iterations; keyed by its length and preconditioner)
   hipGraphExec_t pcg_graph = nullptr;
   int pcg_graph_iters = 0;
+  int pcg_graph_precond = -1;
+
+  // aggregation multigrid preconditioner (built on its first use)
+  Amg amg;
 
   // GMRES workspace (allocated on first use)
   void* d_gm_V = nullptr;   // (m+1) * n KT
@@ -248,6 +285,12 @@ int ensure_bsr(Ctx* c);                                         // d_K from d_Ks
 int launch_tangent_apply_f64(Ctx* c, const double* v_dev, double* y_dev);  // y = K(U) v, FP64
 int launch_locate(Ctx* c, int g, const double* px, const double* py, const double* pz, int64_t np,
                   double tol, int64_t* elem, double* w);
+
+// ---- amg.hip ----
+int amg_build(Ctx* c);                                          // hierarchy (host) + arrays
+int amg_numeric(Ctx* c);                                        // Galerkin, inverses, omega
+int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st);  // x = M^-1 b
+void amg_free(Ctx* c);
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2

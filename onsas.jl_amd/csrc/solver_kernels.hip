@@ -462,6 +462,91 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B(const KT* __restrict__ p, const
   }
 }
 
+// ---- PCG with a separate preconditioner (AMG): the three pieces of pcg_B
+// without the fused Jacobi, and r.z after the preconditioner ------------------------
+// r = b (free dofs, KT), x = 0; partials r.r (and b.b)
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_pcg_init_r(const double* __restrict__ b,
+                                                     const uint8_t* __restrict__ mask,
+                                                     KT* __restrict__ x, KT* __restrict__ r,
+                                                     double* __restrict__ prr,
+                                                     double* __restrict__ pbb, int np, int64_t chunk,
+                                                     int64_t nn) {
+  __shared__ double sh[NT / 64];
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double rr = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    const uint32_t m = mask[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const KT ri = ((m >> d) & 1u) ? KT(0) : (KT)b[3 * i + d];
+      x[3 * i + d] = KT(0);
+      r[3 * i + d] = ri;
+      rr += (double)ri * (double)ri;
+    }
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    prr[blockIdx.x] = rr;
+    pbb[blockIdx.x] = rr;
+  }
+}
+
+// alpha = r.z / p.q; x += alpha p; r -= alpha q; partials r.r; breakdown test
+template <typename KT>
+__global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, const KT* __restrict__ q,
+                                                     KT* __restrict__ x, KT* __restrict__ r,
+                                                     const double* __restrict__ prz_cur,
+                                                     const double* __restrict__ ppq,
+                                                     double* __restrict__ prr, PcgState* __restrict__ st,
+                                                     int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[3 * NT / 64];
+  if (st->done | st->breakdown) return;
+  double rz, pq, unused;
+  sum_parts3(prz_cur, ppq, nullptr, np, sh, rz, pq, unused);
+  if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->breakdown = 1;
+      st->breakdown_k = st->k;
+    }
+    return;
+  }
+  const KT alpha = (KT)(rz / pq);
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double rrn = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[3 * i + d] += alpha * p[3 * i + d];
+      const KT ri = r[3 * i + d] - alpha * q[3 * i + d];
+      r[3 * i + d] = ri;
+      rrn += (double)ri * (double)ri;
+    }
+  }
+  rrn = block_sum(rrn, sh);
+  if (threadIdx.x == 0) {
+    prr[blockIdx.x] = rrn;
+    if (blockIdx.x == 0) st->k = st->k + 1;
+  }
+}
+
+// partials r.z (after the preconditioner)
+template <typename KT>
+__global__ __launch_bounds__(NT) void k_pcg_rz(const KT* __restrict__ r, const KT* __restrict__ z,
+                                                 double* __restrict__ prz, const PcgState* __restrict__ st,
+                                                 int np, int64_t chunk, int64_t nn) {
+  __shared__ double sh[NT / 64];
+  if (st->done | st->breakdown) return;
+  int64_t r0, r1;
+  wg_rows(np, chunk, nn, r0, r1);
+  double rz = 0;
+  for (int64_t i = 3 * r0 + threadIdx.x; i < 3 * r1; i += NT) rz += (double)r[i] * (double)z[i];
+  rz = block_sum(rz, sh);
+  if (threadIdx.x == 0) prz[blockIdx.x] = rz;
+}
+
 // y = K x (FP64 accumulation), for onsas_tangent_matvec
 template <typename KT>
 __global__ __launch_bounds__(256) void k_spmv_f64(const int32_t* __restrict__ row_ptr,
@@ -664,6 +749,15 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
                        (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
                        RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np,
                        c->chunk, c->nn);
+  if (c->lin.precond == ONSAS_PRECOND_AMG) {
+    hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
+                       (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
+                       c->np, c->chunk, c->nn);
+    (void)launch_vcycle(c, c->d_pr, c->d_pz, st);
+    hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
+                       (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
+    return;
+  }
   hipLaunchKernelGGL(k_pcg_B<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                      (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz, RZ[par],
                      part(c, kPartPQ), RZ[1 - par], part(c, kPartRR), st, c->np, c->chunk, c->nn);
@@ -671,7 +765,8 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
 
 template <typename KT>
 int build_pcg_graph(Ctx* c, int iters) {
-  if (c->pcg_graph && c->pcg_graph_iters == iters) return ONSAS_OK;
+  if (c->pcg_graph && c->pcg_graph_iters == iters && c->pcg_graph_precond == c->lin.precond)
+    return ONSAS_OK;
   if (c->pcg_graph) {
     hipGraphExecDestroy(c->pcg_graph);
     c->pcg_graph = nullptr;
@@ -686,6 +781,7 @@ int build_pcg_graph(Ctx* c, int iters) {
   hipGraphDestroy(graph);
   if (e != hipSuccess) return set_err(c, ONSAS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
   c->pcg_graph_iters = iters;
+  c->pcg_graph_precond = c->lin.precond;
   return ONSAS_OK;
 }
 
@@ -698,14 +794,28 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
   const int maxiter = c->lin.maxiter > 0 ? c->lin.maxiter : (int)std::min<int64_t>(c->nfree, 1 << 30);
   int chunk_iters = c->lin.check_every > 0 ? c->lin.check_every : 16;
   chunk_iters += chunk_iters & 1;  // even: the parity ping-pong repeats per chunk
-  hipLaunchKernelGGL(k_pcg_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
-                     (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz,
-                     part(c, kPartRZ0), part(c, kPartRR), part(c, kPartBB), np, c->chunk, c->nn);
+  const bool amg = c->lin.precond == ONSAS_PRECOND_AMG;
+  if (amg) {  // hierarchy on first use, numerics after every assembly
+    if (!c->amg.built) ONSAS_TRY(amg_build(c));
+    if (c->amg.stale) ONSAS_TRY(amg_numeric(c));
+    hipLaunchKernelGGL(k_pcg_init_r<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
+                       (KT*)c->d_px, (KT*)c->d_pr, part(c, kPartRR), part(c, kPartBB), np, c->chunk,
+                       c->nn);
+  } else {
+    hipLaunchKernelGGL(k_pcg_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
+                       (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz,
+                       part(c, kPartRZ0), part(c, kPartRR), part(c, kPartBB), np, c->chunk, c->nn);
+  }
   const bool f = forcing && c->lin.forcing;
   hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(NT), 0, s, part(c, kPartBB),
                      part(c, kPartF2), &c->d_sc->pcg, &c->d_sc->nwt, rtol, maxiter, np, f ? 1 : 0,
                      c->lin.newton_tol, c->lin.eta_max > 0 ? c->lin.eta_max : 0.1,
                      c->lin.eta_first > 0 ? c->lin.eta_first : 1e-3);
+  if (amg) {
+    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg));
+    hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
+                       (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk, c->nn);
+  }
   ONSAS_HIP_TRY(c, hipGetLastError());
   // ONSAS_PCG_GRAPH=0 replays the chunks as direct launches (same kernels, same
   // order); profilers that intercept graph launches need it.

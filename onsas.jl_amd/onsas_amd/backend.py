@@ -18,6 +18,7 @@ from .entities import Tetrahedra, Trusses
 from .materials import IsotropicLinearElastic, NeoHookean, SVK
 
 _SOLVERS = {"auto": _capi.LINSOLVE_AUTO, "pcg": _capi.LINSOLVE_PCG, "gmres": _capi.LINSOLVE_GMRES}
+_PRECONDS = {"block-jacobi": _capi.PRECOND_BLOCK_JACOBI, "amg": _capi.PRECOND_AMG}
 
 
 @dataclass
@@ -25,7 +26,9 @@ class MI355XBackend:
     """precision: "mixed" (FP32 tangent + PCG, FP64 f_int / residual / norms / U)
     or "fp64".  rtol/maxiter: inner-solve stopping rule (None: sqrt(eps) in fp64
     mode as the reference, 1e-6 in mixed mode; maxiter = free dofs).  solver:
-    "auto" (PCG, GMRES if the tangent is indefinite), "pcg" or "gmres"."""
+    "auto" (PCG, GMRES if the tangent is indefinite), "pcg" or "gmres".
+    preconditioner: "block-jacobi" (3x3 nodal blocks) or "amg" (aggregation
+    multigrid V-cycle with rigid-body modes)."""
 
     precision: str = "mixed"
     rtol: float = None
@@ -39,6 +42,7 @@ class MI355XBackend:
     forcing: bool = False
     eta_max: float = 0.1
     eta_first: float = 1e-2
+    preconditioner: str = "block-jacobi"
 
     def precision_code(self):
         if self.precision not in ("mixed", "fp64"):
@@ -95,7 +99,7 @@ class GpuStaticState:
         b = self.backend
         self.ctx.set_linear_solver(_SOLVERS[b.solver], b.rtol or 0.0, b.maxiter or 0,
                                    b.gmres_restart, b.check_every, b.forcing, newton_tol,
-                                   b.eta_max, b.eta_first)
+                                   b.eta_max, b.eta_first, _PRECONDS[b.preconditioner])
 
     # -- loads
     def apply_loads(self, t):

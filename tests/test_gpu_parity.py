@@ -469,3 +469,55 @@ def test_full_size_compression_cube_mixed():
           f"lin iters {sol.linear_iterations()}", flush=True)
     assert err <= 1e-6
     assert all(r["dr_rel"] <= 1e-8 for r in sol.records)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_amg_pcg_matches_direct(oracle, prec):
+    """PCG with the aggregation-multigrid V-cycle vs a sparse direct solve, and
+    fewer iterations than block-Jacobi on the same system."""
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] AMG PCG vs direct {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=8, jitter=0.05, seed=2)
+    fixed = s.fixed_dofs
+    a = workloads.nh_uniaxial_alpha(-p * 0.5, 8.3e5, 3.8e5)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    U = workloads.homogeneous_displacement(sm.coords, a, b)
+    rtol = 1e-10 if prec == "fp64" else 1e-6
+    rhs = np.random.default_rng(3).normal(size=3 * sm.nn)
+    its = {}
+    for pc in (_capi.PRECOND_BLOCK_JACOBI, _capi.PRECOND_AMG):
+        c = _ctx(prec, sm.coords, sm.tets, "nh", fixed=fixed)
+        c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=5000, precond=pc)
+        c.set_U(U)
+        c.assemble()
+        x, it, rr = c.linear_solve(rhs)
+        its[pc] = it
+        if pc == _capi.PRECOND_AMG:
+            Kff = _oracle_model(oracle, sm.coords, sm.tets, "nh", fixed).assemble(
+                U, want_full=False, want_ff=True)["Kff"]
+            free = np.setdiff1d(np.arange(3 * sm.nn), fixed)
+            xo = spla.spsolve(Kff.tocsc(), rhs[free])
+            err = np.abs(x[free] - xo).max() / np.abs(xo).max()
+            assert err < (1e-8 if prec == "fp64" else 2e-4), err
+            assert np.all(x[fixed] == 0)
+            # a second solve reuses the hierarchy; a new assembly refreshes it
+            x2, it2, _ = c.linear_solve(rhs)
+            assert np.array_equal(x, x2) and it2 == it
+        c.close()
+    print(f"  iterations block-Jacobi {its[0]} AMG {its[1]}", flush=True)
+    assert its[_capi.PRECOND_AMG] < 0.5 * its[_capi.PRECOND_BLOCK_JACOBI]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_newton_with_amg(prec):
+    """Compression cube with the AMG preconditioner: the closed-form state."""
+    print(f"[gpu] Newton with AMG {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=6, jitter=0.05, seed=2)
+    nsteps = 4
+    sol = _newton(s, nsteps, prec, preconditioner="amg")
+    for k, t in enumerate(np.linspace(1 / nsteps, 1.0, nsteps)):
+        a = workloads.nh_uniaxial_alpha(-p * t, 8.3e5, 3.8e5)
+        b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+        ex = workloads.homogeneous_displacement(sm.coords, a, b)
+        np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-8 * np.abs(ex).max())
+        assert sol.records[k]["dr_rel"] <= 1e-10
