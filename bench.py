@@ -32,13 +32,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False):
+def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False,
+             precond="block-jacobi"):
     from onsas_amd.analysis import (ConvergenceSettings, NewtonRaphson, NonLinearStaticAnalysis,
                                     ResidualsIterationStep, solve_)
     from onsas_amd.backend import GpuStaticState, MI355XBackend
     from onsas_amd import workloads
 
-    be = MI355XBackend(precision=precision, maxiter=20000, store_stresses=False, forcing=forcing)
+    be = MI355XBackend(precision=precision, maxiter=20000, store_stresses=False, forcing=forcing,
+                       preconditioner=precond)
     t0 = time.perf_counter()
     state = GpuStaticState(s, be)
     setup_s = time.perf_counter() - t0
@@ -137,6 +139,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-fixed", action="store_true", help="skip the fixed-rtol runs")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precond", default="block-jacobi", choices=["block-jacobi", "amg"],
+                    help="PCG preconditioner of every run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this engine has no multi-GPU path); "
@@ -151,15 +155,17 @@ def main():
     runs = {}
     # headline: mixed precision with inexact-Newton forcing
     runs["mixed+forcing"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps,
-                                     forcing=True)
+                                     forcing=True, precond=args.precond)
     if not args.no_fp64:
         runs["fp64+forcing"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps,
-                                        forcing=True)
+                                        forcing=True, precond=args.precond)
     if not args.no_fixed:
         # the fixed inner tolerances: 1e-6 (mixed) and sqrt(eps) as in the reference (fp64)
-        runs["mixed"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps)
+        runs["mixed"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps,
+                                 precond=args.precond)
         if not args.no_fp64:
-            runs["fp64"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps)
+            runs["fp64"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps,
+                                    precond=args.precond)
 
     def asm_rates(r, kt):
         """Assembly throughput and its HBM roofline fraction from algorithmic

@@ -52,7 +52,12 @@ typedef struct onsas_ctx onsas_ctx;
 #define ONSAS_LINSOLVE_GMRES 2 /* restarted GMRES only (block-Jacobi, right precond.) */
 
 #define ONSAS_PRECOND_BLOCK_JACOBI 0 /* 3x3 nodal block-Jacobi                          */
-#define ONSAS_PRECOND_AMG 1          /* aggregation multigrid V-cycle (rigid-body modes) */
+#define ONSAS_PRECOND_AMG 1          /* smoothed-aggregation multigrid V-cycle (rigid-body
+                                        modes, Chebyshev smoothing)                        */
+
+#define ONSAS_AMG_REFRESH_STEP 0     /* multigrid coarse levels rebuilt at the first solve of
+                                        each load step (after onsas_set_loads)             */
+#define ONSAS_AMG_REFRESH_ASSEMBLY 1 /* ... rebuilt after every assembly                   */
 
 /* Linear-solver settings; replace the tolerances of
  * src/StructuralSolvers/StructuralSolvers.jl:229-234 (abstol 0, reltol sqrt(eps),
@@ -72,6 +77,9 @@ typedef struct {
     double eta_first;      /* forcing: inner tolerance of the first Newton iteration of a
                               load step (<= 0: 1e-3)                                      */
     int32_t precond;       /* PCG preconditioner: ONSAS_PRECOND_* (GMRES: block-Jacobi)  */
+    int32_t amg_refresh;   /* ONSAS_AMG_REFRESH_*: when the multigrid coarse levels follow
+                              the tangent (its finest level, the tangent itself, always
+                              does)                                                      */
 } onsas_linear_settings;
 
 /* Per-Newton-iteration record: the four norms of

@@ -1,30 +1,34 @@
-// amg.hip -- aggregation multigrid preconditioner for the PCG (an option beside
-// block-Jacobi; the reference's CG is unpreconditioned).  gfx950, wave64.
+// amg.hip -- smoothed-aggregation multigrid preconditioner for the PCG (an option
+// beside block-Jacobi; the reference's CG is unpreconditioned).  gfx950, wave64.
 //
-// Hierarchy (built once on the host, from the block pattern and the node
-// coordinates):
+// Hierarchy (symbolic part built once on the host from the block pattern and the
+// node coordinates):
 //   * greedy aggregation of the node graph: a node whose neighbours are all
 //     free becomes a root with its neighbours, left-over nodes join a
 //     neighbouring aggregate, isolated ones form their own;
 //   * near-nullspace: the 6 rigid-body modes of each node (zero on fixed dofs),
 //     orthonormalised per aggregate by modified Gram-Schmidt (twice): the
-//     aggregate's rows of Q form the tentative prolongator P (bs x 6 per node),
+//     aggregate's rows of Q form the tentative prolongator T (bs x 6 per node),
 //     R (6 x 6) is the next level's near-nullspace;
+//   * smoothed prolongator P = (I - w D^-1 A) T, w = 4 / (3 lambda_max(D^-1 A))
+//     (block-CSR, bs x 6 blocks: row i has a block for every aggregate holding a
+//     neighbour of i); next level A_c = P^T (A P);
 //   * coarse levels have 6 dofs per node; coarsening goes on until at most 12
 //     nodes (72 dofs) remain, whose matrix is inverted densely in LDS.
-// Numerics (every assembly, on the device): Galerkin A_c = P^T A P summed per
-// coarse block over fixed-order lists of the finer blocks that land in it
-// (deterministic, no atomics), 6x6 block-Jacobi inverses, and the dense
-// inverse of the coarsest matrix.  A coarse dof whose column of Q vanished
-// (rank-deficient aggregate) gets a unit diagonal.
-// V-cycle: one damped block-Jacobi sweep (omega = 1 / (1.1 lambda_max(D^-1 A)),
-// lambda by power iteration) before and after the coarse correction, the
-// coarsest level solved exactly; the cycle is symmetric, so it is a valid PCG
-// preconditioner.  Every cycle kernel returns at once when the PCG state says
-// done, so replayed graphs cost nothing after convergence.
+// Numerics (after every assembly, on the device): P, W = A P and P^T W, each
+// entry an FP64 sum over a fixed-order list built by the host (deterministic,
+// no atomics), 6x6 block-Jacobi inverses, the dense inverse of the coarsest
+// matrix, and lambda_max(D^-1 A) per level by power iteration (15 steps the
+// first time, then 3 steps warm-started from the last iterate).  A coarse dof
+// whose column of T vanished (rank-deficient aggregate) gets a unit diagonal.
+// V-cycle: Chebyshev smoothing of degree 2 with the block-Jacobi inverse on
+// [1.1 lambda / 30, 1.1 lambda] before and after the coarse correction (the same
+// polynomial, so the cycle is symmetric and a valid PCG preconditioner), the
+// coarsest level solved exactly.  Each operator product carries the Chebyshev
+// update of its rows in its epilogue.  Every cycle kernel returns at once when
+// the PCG state says done, so replayed graphs cost nothing after convergence.
 #include <algorithm>
 #include <cmath>
-#include <numeric>
 #include <vector>
 
 #include "engine.hpp"
@@ -34,54 +38,149 @@ namespace onsas {
 namespace {
 
 constexpr int kMaxCoarseNodes = 12;  // coarsest level: <= 72 dofs, inverted in LDS
+constexpr double kChebRatio = 30.0;  // smoothing interval [hi / 30, hi], hi = 1.1 lambda
+constexpr int kPowerFirst = 15;      // power-iteration steps, first estimate
+constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 __device__ inline bool pcg_stopped(const PcgState* st) { return st && (st->done | st->breakdown); }
 
+template <typename T>
+__device__ inline T pick6(const T (&a)[6], int u) {
+  T r = a[0];
+  r = u == 1 ? a[1] : r;
+  r = u == 2 ? a[2] : r;
+  r = u == 3 ? a[3] : r;
+  r = u == 4 ? a[4] : r;
+  r = u == 5 ? a[5] : r;
+  return r;
+}
+
+// Sum of a 6-vector over the 64 lanes of a wave (every lane gets the total).
+template <typename KT>
+__device__ inline void wave_sum6(KT (&v)[6]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] += __shfl_xor(v[q], o, 64);
+}
+
+// Chebyshev degree-2 coefficients for lambda_max(D^-1 A) = lam: the smoother is
+// d0 = D^-1 b / theta, d1 = c1 d0 + c2 D^-1 (b - A d0), S b = d0 + d1.
+struct Cheb {
+  double c1, c2, inv_theta;
+};
+Cheb cheb(double lam) {
+  const double hi = 1.1 * lam, lo = hi / kChebRatio;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+  return {rho1 * rho0, 2.0 * rho1 / delta, 1.0 / theta};
+}
+
 // ---- numeric setup ---------------------------------------------------------------
 
-// One thread per coarse block: sum over its finer blocks (i, j) of P_i^T A_ij P_j,
-// in FP64.  Finer values at off + stride * (row-major entry).
-template <typename KT, int BSF>
-__global__ void k_galerkin(const int64_t* __restrict__ gptr, const int64_t* __restrict__ goff,
-                           const int32_t* __restrict__ gi, const int32_t* __restrict__ gj,
-                           const KT* __restrict__ Af, int64_t fstride, const KT* __restrict__ Pf,
-                           KT* __restrict__ Ac, int64_t ncb) {
-  const int64_t cb = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cb >= ncb) return;
-  double acc[36];
+// P block p = (i, J), one thread per column c: T_i[:, c] if J is the aggregate of
+// i, minus w D_i^-1 times the sum over the operator blocks k of row i whose column
+// j lies in J (list pk, row order) of A_k T_j[:, c].  FP64 sums.
+template <typename KT, int BS>
+__global__ __launch_bounds__(256) void k_sa_prolongator(
+    const int32_t* __restrict__ p_row, const int32_t* __restrict__ p_col,
+    const int32_t* __restrict__ agg, const int64_t* __restrict__ pk_ptr,
+    const int32_t* __restrict__ pk, const int32_t* __restrict__ col, const KT* __restrict__ A,
+    const KT* __restrict__ Dinv, const KT* __restrict__ T, double w, KT* __restrict__ P,
+    int64_t nnz_p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nnz_p) return;
+  const int64_t p = t / 6;
+  const int c = (int)(t % 6);
+  const int64_t i = p_row[p];
+  double s[BS];
 #pragma unroll
-  for (int q = 0; q < 36; ++q) acc[q] = 0;
-  for (int64_t t = gptr[cb]; t < gptr[cb + 1]; ++t) {
-    const KT* a = Af + goff[t];
-    const KT* pi = Pf + (int64_t)gi[t] * (BSF * 6);
-    const KT* pj = Pf + (int64_t)gj[t] * (BSF * 6);
-    double T[BSF][6];  // A_ij P_j
+  for (int r = 0; r < BS; ++r) s[r] = 0;
+  for (int64_t e = pk_ptr[p]; e < pk_ptr[p + 1]; ++e) {
+    const int64_t k = pk[e];
+    const KT* a = A + (int64_t)BS * BS * k;
+    const KT* tj = T + (int64_t)BS * 6 * col[k] + c;
+    double tc[BS];
 #pragma unroll
-    for (int r = 0; r < BSF; ++r)
+    for (int q = 0; q < BS; ++q) tc[q] = (double)tj[6 * q];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) T[r][c] = 0;
+    for (int r = 0; r < BS; ++r)
 #pragma unroll
-    for (int r = 0; r < BSF; ++r)
-#pragma unroll
-      for (int s = 0; s < BSF; ++s) {
-        const double v = (double)a[fstride * (r * BSF + s)];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) T[r][c] += v * (double)pj[s * 6 + c];
-      }
-#pragma unroll
-    for (int r = 0; r < BSF; ++r)
-#pragma unroll
-      for (int u = 0; u < 6; ++u) {
-        const double pv = (double)pi[r * 6 + u];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[u * 6 + c] += pv * T[r][c];
-      }
+      for (int q = 0; q < BS; ++q) s[r] += (double)a[BS * r + q] * tc[q];
   }
-  KT* o = Ac + 36 * cb;
+  const KT* d = Dinv + (int64_t)BS * BS * i;
+  const bool own = p_col[p] == agg[i];
+  const KT* ti = T + (int64_t)BS * 6 * i + c;
+  KT* o = P + (int64_t)BS * 6 * p + c;
 #pragma unroll
-  for (int q = 0; q < 36; ++q) o[q] = (KT)acc[q];
+  for (int r = 0; r < BS; ++r) {
+    double ds = 0;
+#pragma unroll
+    for (int m = 0; m < BS; ++m) ds += (double)d[BS * r + m] * s[m];
+    o[6 * r] = (KT)((own ? (double)ti[6 * r] : 0.0) - w * ds);
+  }
+}
+
+// W block (i, J) = (A P)_iJ, one thread per column c: sum over its (k, p) pairs
+// (list order) of A_k P_p[:, c].
+template <typename KT, int BS>
+__global__ __launch_bounds__(256) void k_sa_ap(const int64_t* __restrict__ wl_ptr,
+                                               const int32_t* __restrict__ wl_k,
+                                               const int32_t* __restrict__ wl_p,
+                                               const KT* __restrict__ A, const KT* __restrict__ P,
+                                               KT* __restrict__ W, int64_t nnz_w) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nnz_w) return;
+  const int64_t w = t / 6;
+  const int c = (int)(t % 6);
+  double s[BS];
+#pragma unroll
+  for (int r = 0; r < BS; ++r) s[r] = 0;
+  for (int64_t e = wl_ptr[w]; e < wl_ptr[w + 1]; ++e) {
+    const KT* a = A + (int64_t)BS * BS * wl_k[e];
+    const KT* pc = P + (int64_t)BS * 6 * wl_p[e] + c;
+    double pv[BS];
+#pragma unroll
+    for (int q = 0; q < BS; ++q) pv[q] = (double)pc[6 * q];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+      for (int q = 0; q < BS; ++q) s[r] += (double)a[BS * r + q] * pv[q];
+  }
+  KT* o = W + (int64_t)BS * 6 * w + c;
+#pragma unroll
+  for (int r = 0; r < BS; ++r) o[6 * r] = (KT)s[r];
+}
+
+// Next-level block g = (I, J) = (P^T W)_IJ, one thread per column c: sum over its
+// (p, w) pairs (list order) of P_p^T W_w[:, c].
+template <typename KT, int BS>
+__global__ __launch_bounds__(256) void k_sa_ptap(const int64_t* __restrict__ gl_ptr,
+                                                 const int32_t* __restrict__ gl_p,
+                                                 const int32_t* __restrict__ gl_w,
+                                                 const KT* __restrict__ P, const KT* __restrict__ W,
+                                                 KT* __restrict__ Ac, int64_t nnzb_c) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nnzb_c) return;
+  const int64_t g = t / 6;
+  const int c = (int)(t % 6);
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t e = gl_ptr[g]; e < gl_ptr[g + 1]; ++e) {
+    const KT* pp = P + (int64_t)BS * 6 * gl_p[e];
+    const KT* wc = W + (int64_t)BS * 6 * gl_w[e] + c;
+    double wv[BS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r) wv[r] = (double)wc[6 * r];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+      for (int u = 0; u < 6; ++u) s[u] += (double)pp[6 * r + u] * wv[r];
+  }
+  KT* o = Ac + 36 * g + c;
+#pragma unroll
+  for (int u = 0; u < 6; ++u) o[6 * u] = (KT)s[u];
 }
 
 // Diagonal blocks: unit diagonal for unused dofs (written back into A), then the
@@ -178,16 +277,19 @@ __global__ void k_to_kt(const double* __restrict__ s, KT* __restrict__ d, int64_
 }
 
 // ---- V-cycle kernels (all skip once the PCG has stopped) ---------------------------
+//
+// The operator kernels compute r = b - A v (b null: zero) and then, per row,
+//   r_out = r;  d = c1 v + c2 D^-1 r -> d_out;  x_out = x_base + v + d
+// (each output skipped when null, x_base null: zero).  b may be r_out and x_base
+// may be x_out: every row reads them before it writes them.
 
-// r = b - K x on the fine sliced-ELL tangent (one wave per slice).
+// Fine level on the sliced-ELL tangent (one wave per slice).
 template <typename KT>
-__global__ __launch_bounds__(256) void k_sell_resid(const int32_t* __restrict__ sell_ptr,
-                                                    const int32_t* __restrict__ slice_rounds,
-                                                    int64_t rstride, const int32_t* __restrict__ scol,
-                                                    const KT* __restrict__ sval,
-                                                    const KT* __restrict__ x, const KT* __restrict__ b,
-                                                    KT* __restrict__ r, int64_t nslices, int64_t nn,
-                                                    const PcgState* st) {
+__global__ __launch_bounds__(256) void k_sell_cheb(
+    const int32_t* __restrict__ sell_ptr, const int32_t* __restrict__ slice_rounds, int64_t rstride,
+    const int32_t* __restrict__ scol, const KT* __restrict__ sval, const KT* __restrict__ v,
+    const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out,
+    const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st) {
   if (pcg_stopped(st)) return;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -200,11 +302,11 @@ __global__ __launch_bounds__(256) void k_sell_resid(const int32_t* __restrict__ 
   KT a0 = 0, a1 = 0, a2 = 0;
   for (int m = 0; m < rounds; ++m) {
     const uint32_t j = 3u * (uint32_t)cp[rs * m];
-    const KT* v = vp + 9u * rs * m;
-    const KT p0 = x[j], p1 = x[j + 1], p2 = x[j + 2];
-    a0 += v[0] * p0 + v[64] * p1 + v[128] * p2;
-    a1 += v[192] * p0 + v[256] * p1 + v[320] * p2;
-    a2 += v[384] * p0 + v[448] * p1 + v[512] * p2;
+    const KT* vv = vp + 9u * rs * m;
+    const KT p0 = v[j], p1 = v[j + 1], p2 = v[j + 2];
+    a0 += vv[0] * p0 + vv[64] * p1 + vv[128] * p2;
+    a1 += vv[192] * p0 + vv[256] * p1 + vv[320] * p2;
+    a2 += vv[384] * p0 + vv[448] * p1 + vv[512] * p2;
   }
 #pragma unroll
   for (int o = kSellRows; o < 64; o <<= 1) {
@@ -213,38 +315,70 @@ __global__ __launch_bounds__(256) void k_sell_resid(const int32_t* __restrict__ 
     a2 += __shfl_xor(a2, o, 64);
   }
   const int64_t i = kSellRows * s + lane % kSellRows;
-  if (lane < kSellRows && i < nn) {
-    r[3 * i] = b[3 * i] - a0;
-    r[3 * i + 1] = b[3 * i + 1] - a1;
-    r[3 * i + 2] = b[3 * i + 2] - a2;
-  }
-}
-
-// r = b - A x on a 6x6 block-CSR level: one thread per (row, component).
-template <typename KT>
-__global__ void k_bsr6_resid(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                             const KT* __restrict__ A, const KT* __restrict__ x,
-                             const KT* __restrict__ b, KT* __restrict__ r, int64_t n,
-                             const PcgState* st) {
-  if (pcg_stopped(st)) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 6 * n) return;
-  const int64_t I = t / 6;
-  const int u = (int)(t % 6);
-  KT acc = 0;
-  for (int32_t k = row_ptr[I]; k < row_ptr[I + 1]; ++k) {
-    const KT* a = A + 36 * (int64_t)k + 6 * u;
-    const KT* xj = x + 6 * (int64_t)col[k];
+  if (lane >= kSellRows || i >= nn) return;
+  KT r[3] = {-a0, -a1, -a2};
+  if (b)
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc += a[c] * xj[c];
+    for (int q = 0; q < 3; ++q) r[q] += b[3 * i + q];
+  if (r_out)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r_out[3 * i + q] = r[q];
+  if (!d_out && !x_out) return;
+  const KT* dm = Dinv + 9 * i;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const KT vq = v[3 * i + q];
+    const KT d = c1 * vq + c2 * (dm[3 * q] * r[0] + dm[3 * q + 1] * r[1] + dm[3 * q + 2] * r[2]);
+    if (d_out) d_out[3 * i + q] = d;
+    if (x_out) x_out[3 * i + q] = (x_base ? x_base[3 * i + q] : KT(0)) + vq + d;
   }
-  r[t] = b[t] - acc;
 }
 
-// x = omega D^-1 b (first) or x += omega D^-1 r, one thread per (node, component).
+// Levels >= 1 (6x6 block-CSR): one wave per block row, lanes over its blocks,
+// lanes 0..5 finish one component each.
+template <typename KT>
+__global__ __launch_bounds__(256) void k_bsr6_cheb(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col, const KT* __restrict__ A,
+    const KT* __restrict__ v, const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2,
+    KT* __restrict__ d_out, const KT* x_base, KT* x_out, int64_t n, const PcgState* st) {
+  if (pcg_stopped(st)) return;
+  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int l = threadIdx.x & 63;
+  if (I >= n) return;
+  KT acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int32_t k = row_ptr[I] + l; k < row_ptr[I + 1]; k += 64) {
+    const KT* a = A + 36 * (int64_t)k;
+    const KT* xj = v + 6 * (int64_t)col[k];
+    KT xv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xv[c] = xj[c];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[u] += a[6 * u + c] * xv[c];
+  }
+  wave_sum6(acc);
+  if (l >= 6) return;
+  KT r[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) r[m] = (b ? b[6 * I + m] : KT(0)) - acc[m];
+  const KT ru = pick6(r, l);
+  if (r_out) r_out[6 * I + l] = ru;
+  if (!d_out && !x_out) return;
+  const KT* dm = Dinv + 36 * I + 6 * l;
+  KT dr = 0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) dr += dm[m] * r[m];
+  const KT vu = v[6 * I + l];
+  const KT d = c1 * vu + c2 * dr;
+  if (d_out) d_out[6 * I + l] = d;
+  if (x_out) x_out[6 * I + l] = (x_base ? x_base[6 * I + l] : KT(0)) + vu + d;
+}
+
+// y = s D^-1 v, one thread per (node, component).
 template <typename KT, int BS>
-__global__ void k_smooth(const KT* __restrict__ Dinv, const KT* __restrict__ v, KT* __restrict__ x,
-                         int64_t n, KT omega, int accumulate, const PcgState* st) {
+__global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, KT* __restrict__ y,
+                         int64_t n, KT s, const PcgState* st) {
   if (pcg_stopped(st)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= BS * n) return;
@@ -252,48 +386,69 @@ __global__ void k_smooth(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
   const int u = (int)(t % BS);
   const KT* d = Dinv + (int64_t)BS * BS * i + BS * u;
   const KT* vi = v + (int64_t)BS * i;
-  KT s = 0;
-#pragma unroll
-  for (int c = 0; c < BS; ++c) s += d[c] * vi[c];
-  x[t] = accumulate ? x[t] + omega * s : omega * s;
-}
-
-// b_c[I] = sum over the member nodes i of aggregate I of P_i^T r_i.
-template <typename KT, int BSF>
-__global__ void k_restrict(const int64_t* __restrict__ aptr, const int32_t* __restrict__ anodes,
-                           const KT* __restrict__ P, const KT* __restrict__ rf, KT* __restrict__ bc,
-                           int64_t nc, const PcgState* st) {
-  if (pcg_stopped(st)) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 6 * nc) return;
-  const int64_t I = t / 6;
-  const int u = (int)(t % 6);
   KT acc = 0;
-  for (int64_t k = aptr[I]; k < aptr[I + 1]; ++k) {
-    const int64_t i = anodes[k];
-    const KT* p = P + (int64_t)BSF * 6 * i;
 #pragma unroll
-    for (int a = 0; a < BSF; ++a) acc += p[a * 6 + u] * rf[BSF * i + a];
-  }
-  bc[t] = acc;
+  for (int c = 0; c < BS; ++c) acc += d[c] * vi[c];
+  y[t] = s * acc;
 }
 
-// x_f[i] += P_i x_c[agg(i)].
-template <typename KT, int BSF>
-__global__ void k_prolong(const int32_t* __restrict__ agg, const KT* __restrict__ P,
-                          const KT* __restrict__ xc, KT* __restrict__ xf, int64_t nf,
-                          const PcgState* st) {
+// b_c[I] = sum over the P blocks p of column I (list rt, row order) of
+// P_p^T r_row(p): a wave per aggregate, lanes over its blocks.  With d0 non-null
+// also d0[I] = s D_c^-1 b_c[I] (the first Chebyshev step of the coarse level).
+template <typename KT, int BS>
+__global__ __launch_bounds__(256) void k_sa_restrict(
+    const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt,
+    const int32_t* __restrict__ p_row, const KT* __restrict__ P, const KT* __restrict__ rf,
+    KT* __restrict__ bc, const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc,
+    const PcgState* st) {
+  if (pcg_stopped(st)) return;
+  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int l = threadIdx.x & 63;
+  if (I >= nc) return;
+  KT acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int32_t e = rt_ptr[I] + l; e < rt_ptr[I + 1]; e += 64) {
+    const int32_t p = rt[e];
+    const int64_t i = p_row[p];
+    const KT* pp = P + (int64_t)BS * 6 * p;
+    KT rv[BS];
+#pragma unroll
+    for (int a = 0; a < BS; ++a) rv[a] = rf[BS * i + a];
+#pragma unroll
+    for (int a = 0; a < BS; ++a)
+#pragma unroll
+      for (int u = 0; u < 6; ++u) acc[u] += pp[6 * a + u] * rv[a];
+  }
+  wave_sum6(acc);
+  if (l >= 6) return;
+  bc[6 * I + l] = pick6(acc, l);
+  if (d0) {
+    const KT* dm = Dinv_c + 36 * I + 6 * l;
+    KT dr = 0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) dr += dm[m] * acc[m];
+    d0[6 * I + l] = s * dr;
+  }
+}
+
+// x_f[i] += sum over the P blocks p of row i of P_p x_c[col(p)], one thread per
+// (node, dof).
+template <typename KT, int BS>
+__global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
+                             const KT* __restrict__ P, const KT* __restrict__ xc,
+                             KT* __restrict__ xf, int64_t nf, const PcgState* st) {
   if (pcg_stopped(st)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= BSF * nf) return;
-  const int64_t i = t / BSF;
-  const int a = (int)(t % BSF);
-  const KT* p = P + (int64_t)BSF * 6 * i + 6 * a;
-  const KT* xi = xc + 6 * (int64_t)agg[i];
-  KT s = 0;
+  if (t >= BS * nf) return;
+  const int64_t i = t / BS;
+  const int a = (int)(t % BS);
+  KT acc = 0;
+  for (int32_t p = p_ptr[i]; p < p_ptr[i + 1]; ++p) {
+    const KT* pp = P + (int64_t)BS * 6 * p + 6 * a;
+    const KT* xj = xc + 6 * (int64_t)p_col[p];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) s += p[c] * xi[c];
-  xf[t] += s;
+    for (int u = 0; u < 6; ++u) acc += pp[u] * xj[u];
+  }
+  xf[t] += acc;
 }
 
 // x = Ainv b on the coarsest level (dense), one thread per row.
@@ -324,10 +479,11 @@ __global__ __launch_bounds__(256) void k_norm2(const KT* __restrict__ v, int64_t
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// v = a w
 template <typename KT>
-__global__ void k_scale(KT* __restrict__ v, int64_t n, KT a) {
+__global__ void k_scaled_copy(const KT* __restrict__ w, KT* __restrict__ v, int64_t n, KT a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] *= a;
+  if (i < n) v[i] = a * w[i];
 }
 
 template <typename KT>
@@ -360,6 +516,12 @@ int alloc(Ctx* c, void** d, size_t bytes) {
   return ONSAS_OK;
 }
 
+template <typename KT>
+int upload_kt(Ctx* c, void** d, const std::vector<double>& h) {
+  std::vector<KT> v(h.begin(), h.end());
+  return upload(c, (KT**)d, v);
+}
+
 // greedy aggregation of a node graph (CSR, self included)
 int64_t aggregate(const std::vector<int32_t>& rp, const std::vector<int32_t>& cl, int64_t n,
                   std::vector<int32_t>& agg) {
@@ -386,32 +548,95 @@ int64_t aggregate(const std::vector<int32_t>& rp, const std::vector<int32_t>& cl
   return na;
 }
 
+// Tentative prolongator of one level: per aggregate, modified Gram-Schmidt
+// (twice) of the rows of its member nodes in the near-nullspace B (bs x 6 per
+// node); T gets Q, Bc (6 x 6 per aggregate) gets R.
+void tentative(const std::vector<int64_t>& aptr, const std::vector<int32_t>& anodes, int bs,
+               const std::vector<double>& B, std::vector<double>& T, std::vector<double>& Bc) {
+  const int64_t na = (int64_t)aptr.size() - 1;
+  std::vector<double> V, Q, col;
+  for (int64_t a = 0; a < na; ++a) {
+    const int64_t m = aptr[a + 1] - aptr[a];
+    const int64_t rows = m * bs;
+    V.assign(rows * 6, 0.0);
+    Q.assign(rows * 6, 0.0);
+    for (int64_t k = 0; k < m; ++k)
+      for (int r = 0; r < bs; ++r)
+        for (int q = 0; q < 6; ++q) V[(k * bs + r) * 6 + q] = B[anodes[aptr[a] + k] * bs * 6 + r * 6 + q];
+    double R[36] = {0};
+    for (int q = 0; q < 6; ++q) {
+      col.assign(rows, 0.0);
+      double n0 = 0;
+      for (int64_t t = 0; t < rows; ++t) {
+        col[t] = V[t * 6 + q];
+        n0 += col[t] * col[t];
+      }
+      n0 = std::sqrt(n0);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int j = 0; j < q; ++j) {
+          double d = 0;
+          for (int64_t t = 0; t < rows; ++t) d += Q[t * 6 + j] * col[t];
+          R[j * 6 + q] += d;
+          for (int64_t t = 0; t < rows; ++t) col[t] -= d * Q[t * 6 + j];
+        }
+      double nv = 0;
+      for (int64_t t = 0; t < rows; ++t) nv += col[t] * col[t];
+      nv = std::sqrt(nv);
+      if (nv > 1e-10 * n0 && nv > 0) {
+        R[q * 6 + q] = nv;
+        for (int64_t t = 0; t < rows; ++t) Q[t * 6 + q] = col[t] / nv;
+      }
+    }
+    for (int64_t k = 0; k < m; ++k)
+      for (int r = 0; r < bs; ++r)
+        for (int q = 0; q < 6; ++q) T[anodes[aptr[a] + k] * bs * 6 + r * 6 + q] = Q[(k * bs + r) * 6 + q];
+    for (int q = 0; q < 36; ++q) Bc[a * 36 + q] = R[q];
+  }
+}
+
+struct Trip {  // (column key, two list entries)
+  int32_t key, a, b;
+};
+
+// Group the entries of one row by key (stable: generation order within a key):
+// appends the distinct keys to cols and each group's entries to la/lb, closing
+// each group in ptr.
+void group_row(std::vector<Trip>& trips, std::vector<int32_t>& cols, std::vector<int64_t>& ptr,
+               std::vector<int32_t>& la, std::vector<int32_t>* lb) {
+  std::stable_sort(trips.begin(), trips.end(), [](const Trip& x, const Trip& y) { return x.key < y.key; });
+  for (size_t t = 0; t < trips.size(); ++t) {
+    if (t > 0 && trips[t].key != trips[t - 1].key) ptr.push_back((int64_t)la.size());
+    if (t == 0 || trips[t].key != trips[t - 1].key) cols.push_back(trips[t].key);
+    la.push_back(trips[t].a);
+    if (lb) lb->push_back(trips[t].b);
+  }
+  if (!trips.empty()) ptr.push_back((int64_t)la.size());
+}
+
 }  // namespace
 
-// Build the hierarchy (host) and allocate its device arrays.
+// Build the hierarchy's symbolic part (host) and allocate its device arrays.
 int amg_build(Ctx* c) {
   Amg& A = c->amg;
   A = Amg{};
   const int64_t nn = c->nn;
-  // level 0: the fine tangent (3 dofs per node); its near-nullspace
+  const bool f64 = c->precision == ONSAS_PRECISION_FP64;
   AmgLevel L0;
   L0.bs = 3;
   L0.n = nn;
+  L0.nnzb = (int64_t)c->col.size();
   A.lev.push_back(L0);
-  std::vector<double> B(nn * 3 * 6, 0.0);  // node-major, bs x 6
-  for (int64_t i = 0; i < nn; ++i) {
-    double* b = &B[i * 18];
-    for (int d = 0; d < 3; ++d) b[d * 6 + d] = 1.0;
-  }
-  // graph of the current level
+  // level-0 near-nullspace: translations now, rotations per aggregate below
+  std::vector<double> Bcur(nn * 3 * 6, 0.0);
+  for (int64_t i = 0; i < nn; ++i)
+    for (int d = 0; d < 3; ++d) Bcur[i * 18 + d * 6 + d] = 1.0;
   std::vector<int32_t> rp(c->row_ptr), cl(c->col);
   int bs = 3;
   int64_t n = nn;
-  std::vector<double> Bcur = B;  // rotations filled per aggregate (relative to its centroid)
+  std::vector<Trip> trips;
   for (int lev = 0;; ++lev) {
     std::vector<int32_t> agg;
     const int64_t na = aggregate(rp, cl, n, agg);
-    // members per aggregate
     std::vector<int64_t> aptr(na + 1, 0);
     for (int64_t i = 0; i < n; ++i) aptr[agg[i] + 1]++;
     for (int64_t a = 0; a < na; ++a) aptr[a + 1] += aptr[a];
@@ -452,131 +677,108 @@ int amg_build(Ctx* c) {
         }
       }
     }
-    // per aggregate: Q (rows of the member nodes) and R (the coarse near-nullspace)
-    std::vector<double> P(n * bs * 6, 0.0), Bc(na * 36, 0.0);
-    std::vector<double> col;
-    for (int64_t a = 0; a < na; ++a) {
-      const int64_t m = aptr[a + 1] - aptr[a];
-      const int64_t rows = m * bs;
-      std::vector<double> V(rows * 6), Q(rows * 6, 0.0);
-      for (int64_t k = 0; k < m; ++k)
-        for (int r = 0; r < bs; ++r)
-          for (int q = 0; q < 6; ++q) V[(k * bs + r) * 6 + q] = Bcur[anodes[aptr[a] + k] * bs * 6 + r * 6 + q];
-      double R[36] = {0};
-      for (int q = 0; q < 6; ++q) {
-        col.assign(rows, 0.0);
-        double n0 = 0;
-        for (int64_t t = 0; t < rows; ++t) {
-          col[t] = V[t * 6 + q];
-          n0 += col[t] * col[t];
-        }
-        n0 = std::sqrt(n0);
-        for (int pass = 0; pass < 2; ++pass)
-          for (int j = 0; j < q; ++j) {
-            double d = 0;
-            for (int64_t t = 0; t < rows; ++t) d += Q[t * 6 + j] * col[t];
-            R[j * 6 + q] += d;
-            for (int64_t t = 0; t < rows; ++t) col[t] -= d * Q[t * 6 + j];
-          }
-        double nv = 0;
-        for (int64_t t = 0; t < rows; ++t) nv += col[t] * col[t];
-        nv = std::sqrt(nv);
-        if (nv > 1e-10 * n0 && nv > 0) {
-          R[q * 6 + q] = nv;
-          for (int64_t t = 0; t < rows; ++t) Q[t * 6 + q] = col[t] / nv;
-        }
-      }
-      for (int64_t k = 0; k < m; ++k)
-        for (int r = 0; r < bs; ++r)
-          for (int q = 0; q < 6; ++q) P[anodes[aptr[a] + k] * bs * 6 + r * 6 + q] = Q[(k * bs + r) * 6 + q];
-      for (int q = 0; q < 36; ++q) Bc[a * 36 + q] = R[q];
+    std::vector<double> T(n * bs * 6, 0.0), Bc(na * 36, 0.0);
+    tentative(aptr, anodes, bs, Bcur, T, Bc);
+
+    // P pattern (row i: the aggregates of its neighbours) and its lists: the
+    // operator blocks of row i per aggregate, in row order
+    std::vector<int32_t> p_ptr(n + 1, 0), p_col, p_row, pk;
+    std::vector<int64_t> pk_ptr(1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      trips.clear();
+      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) trips.push_back({agg[cl[k]], k, 0});
+      const size_t before = p_col.size();
+      group_row(trips, p_col, pk_ptr, pk, nullptr);
+      p_row.insert(p_row.end(), p_col.size() - before, (int32_t)i);
+      p_ptr[i + 1] = (int32_t)p_col.size();
     }
-    // coarse pattern and Galerkin lists (finer blocks in their storage order)
-    const int64_t nb_f = rp[n];
-    std::vector<std::vector<int32_t>> crow(na);
-    for (int64_t i = 0; i < n; ++i)
-      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) crow[agg[i]].push_back(agg[cl[k]]);
+    const int64_t nnz_p = (int64_t)p_col.size();
+    // W = A P pattern (row i: the P columns of its neighbours) and its lists of
+    // (operator block, P block) pairs, in row and P order
+    std::vector<int32_t> w_ptr(n + 1, 0), w_col, wl_k, wl_p;
+    std::vector<int64_t> wl_ptr(1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      trips.clear();
+      for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+        for (int32_t p = p_ptr[cl[k]]; p < p_ptr[cl[k] + 1]; ++p) trips.push_back({p_col[p], k, p});
+      group_row(trips, w_col, wl_ptr, wl_k, &wl_p);
+      w_ptr[i + 1] = (int32_t)w_col.size();
+    }
+    const int64_t nnz_w = (int64_t)w_col.size();
+    // P^T: per aggregate its P blocks, in row order
+    std::vector<int32_t> rt_ptr(na + 1, 0), rt(nnz_p);
+    for (int64_t p = 0; p < nnz_p; ++p) rt_ptr[p_col[p] + 1]++;
+    for (int64_t a = 0; a < na; ++a) rt_ptr[a + 1] += rt_ptr[a];
+    {
+      std::vector<int32_t> pos(rt_ptr.begin(), rt_ptr.end() - 1);
+      for (int64_t p = 0; p < nnz_p; ++p) rt[pos[p_col[p]]++] = (int32_t)p;
+    }
+    // next level: pattern of P^T W and its lists of (P block, W block) pairs
     AmgLevel Lc;
     Lc.bs = 6;
     Lc.n = na;
     Lc.row_ptr.assign(na + 1, 0);
-    for (int64_t a = 0; a < na; ++a) {
-      std::sort(crow[a].begin(), crow[a].end());
-      crow[a].erase(std::unique(crow[a].begin(), crow[a].end()), crow[a].end());
-      Lc.row_ptr[a + 1] = Lc.row_ptr[a] + (int32_t)crow[a].size();
-    }
-    Lc.nnzb = Lc.row_ptr[na];
-    Lc.col.resize(Lc.nnzb);
-    Lc.diag.assign(na, -1);
-    for (int64_t a = 0; a < na; ++a) {
-      std::copy(crow[a].begin(), crow[a].end(), Lc.col.begin() + Lc.row_ptr[a]);
-      auto it = std::lower_bound(crow[a].begin(), crow[a].end(), (int32_t)a);
-      Lc.diag[a] = Lc.row_ptr[a] + (int32_t)(it - crow[a].begin());
-    }
-    std::vector<int64_t> gptr(Lc.nnzb + 1, 0), goff(nb_f);
-    std::vector<int32_t> gi(nb_f), gj(nb_f);
-    std::vector<int32_t> cblk(nb_f);
-    for (int64_t i = 0; i < n; ++i)
-      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-        const int32_t I = agg[i], J = agg[cl[k]];
-        auto b0 = Lc.col.begin() + Lc.row_ptr[I], b1 = Lc.col.begin() + Lc.row_ptr[I + 1];
-        cblk[k] = (int32_t)(std::lower_bound(b0, b1, J) - Lc.col.begin());
-        gptr[cblk[k] + 1]++;
+    std::vector<int32_t> gl_p, gl_w, diag(na, -1);
+    std::vector<int64_t> gl_ptr(1, 0);
+    for (int64_t I = 0; I < na; ++I) {
+      trips.clear();
+      for (int32_t e = rt_ptr[I]; e < rt_ptr[I + 1]; ++e) {
+        const int32_t p = rt[e], i = p_row[p];
+        for (int32_t w = w_ptr[i]; w < w_ptr[i + 1]; ++w) trips.push_back({w_col[w], p, w});
       }
-    for (int64_t b = 0; b < Lc.nnzb; ++b) gptr[b + 1] += gptr[b];
-    {
-      std::vector<int64_t> pos(gptr.begin(), gptr.end() - 1);
-      const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
-      for (int64_t i = 0; i < n; ++i)
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-          const int64_t t = pos[cblk[k]]++;
-          if (lev == 0) {  // the finer block lives in the sliced-ELL copy
-            const int64_t s = i / kSellRows, kk = k - rp[i];
-            const int64_t lane = kSellRows * (kk % kSellLanes) + i % kSellRows;
-            goff[t] = 9 * (c->sell_ptr[s] + rstride * (kk / kSellLanes)) + lane;
-          } else {
-            goff[t] = 36 * (int64_t)k;
-          }
-          gi[t] = (int32_t)i;
-          gj[t] = cl[k];
-        }
+      const size_t before = Lc.col.size();
+      group_row(trips, Lc.col, gl_ptr, gl_p, &gl_w);
+      for (size_t k = before; k < Lc.col.size(); ++k)
+        if (Lc.col[k] == I) diag[I] = (int32_t)k;
+      Lc.row_ptr[I + 1] = (int32_t)Lc.col.size();
+      if (diag[I] < 0)
+        return set_err(c, ONSAS_ERR_UNSUPPORTED, "multigrid: coarse node without a diagonal block");
     }
-    // device arrays: transfer of the finer level, Galerkin lists of the coarse one
+    Lc.nnzb = (int64_t)Lc.col.size();
+
+    // device arrays of the transfer (finer level) and of the coarse level
     AmgLevel& F = A.lev.back();
-    std::vector<int32_t> agg32(agg.begin(), agg.end());
-    ONSAS_TRY(upload(c, &F.d_agg, agg32));
-    ONSAS_TRY(upload(c, &F.d_aptr, aptr));
-    ONSAS_TRY(upload(c, &F.d_anodes, anodes));
-    {
-      std::vector<float> Pf(P.begin(), P.end());
-      if (c->precision == ONSAS_PRECISION_FP64) ONSAS_TRY(upload(c, (double**)&F.d_P, P));
-      else ONSAS_TRY(upload(c, (float**)&F.d_P, Pf));
-    }
-    ONSAS_TRY(upload(c, &Lc.d_gptr, gptr));
-    ONSAS_TRY(upload(c, &Lc.d_goff, goff));
-    ONSAS_TRY(upload(c, &Lc.d_gi, gi));
-    ONSAS_TRY(upload(c, &Lc.d_gj, gj));
+    F.nnz_p = nnz_p;
+    F.nnz_w = nnz_w;
+    ONSAS_TRY(upload(c, &F.d_agg, agg));
+    ONSAS_TRY(f64 ? upload_kt<double>(c, &F.d_T, T) : upload_kt<float>(c, &F.d_T, T));
+    ONSAS_TRY(upload(c, &F.d_p_ptr, p_ptr));
+    ONSAS_TRY(upload(c, &F.d_p_col, p_col));
+    ONSAS_TRY(upload(c, &F.d_p_row, p_row));
+    ONSAS_TRY(upload(c, &F.d_pk_ptr, pk_ptr));
+    ONSAS_TRY(upload(c, &F.d_pk, pk));
+    ONSAS_TRY(upload(c, &F.d_rt_ptr, rt_ptr));
+    ONSAS_TRY(upload(c, &F.d_rt, rt));
+    ONSAS_TRY(upload(c, &F.d_wl_ptr, wl_ptr));
+    ONSAS_TRY(upload(c, &F.d_wl_k, wl_k));
+    ONSAS_TRY(upload(c, &F.d_wl_p, wl_p));
+    ONSAS_TRY(upload(c, &F.d_gl_ptr, gl_ptr));
+    ONSAS_TRY(upload(c, &F.d_gl_p, gl_p));
+    ONSAS_TRY(upload(c, &F.d_gl_w, gl_w));
+    ONSAS_TRY(alloc(c, &F.d_P, c->kt * bs * 6 * (size_t)nnz_p));
+    ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)nnz_w));
     ONSAS_TRY(upload(c, &Lc.d_row_ptr, Lc.row_ptr));
     ONSAS_TRY(upload(c, &Lc.d_col, Lc.col));
-    ONSAS_TRY(upload(c, &Lc.d_diag, Lc.diag));
+    ONSAS_TRY(upload(c, &Lc.d_diag, diag));
     ONSAS_TRY(alloc(c, &Lc.d_A, c->kt * 36 * (size_t)Lc.nnzb));
     ONSAS_TRY(alloc(c, &Lc.d_Dinv, c->kt * 36 * (size_t)na));
-    for (void** v : {&Lc.d_b, &Lc.d_x, &Lc.d_r}) ONSAS_TRY(alloc(c, v, c->kt * 6 * (size_t)na));
+    for (void** v : {&Lc.d_b, &Lc.d_x, &Lc.d_r, &Lc.d_d0, &Lc.d_d1, &Lc.d_ev})
+      ONSAS_TRY(alloc(c, v, c->kt * 6 * (size_t)na));
     const bool last = na <= kMaxCoarseNodes;
     if (!last && (double)na > (double)n / 1.05)
       return set_err(c, ONSAS_ERR_UNSUPPORTED,
                      "multigrid coarsening stalled at " + std::to_string(na) + " nodes");
     A.lev.push_back(std::move(Lc));
     if (last) break;
-    // the coarse level becomes the finer one
     rp = A.lev.back().row_ptr;
     cl = A.lev.back().col;
     n = na;
     bs = 6;
     Bcur.swap(Bc);
   }
-  ONSAS_TRY(alloc(c, &A.lev[0].d_r, c->kt * 3 * (size_t)nn));
-  A.nc = 6 * A.lev.back().n;
+  for (void** v : {&A.lev[0].d_x, &A.lev[0].d_r, &A.lev[0].d_d0, &A.lev[0].d_d1, &A.lev[0].d_ev})
+    ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));
+  A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, &A.d_Ainv, c->kt * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, (void**)&A.d_part, sizeof(double) * 1024));
@@ -587,62 +789,42 @@ int amg_build(Ctx* c) {
 
 namespace {
 
+// r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
 template <typename KT>
-int apply_level_op(Ctx* c, int l, const KT* x, const KT* b, KT* r, const PcgState* st) {
-  Amg& A = c->amg;
+void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
+             const KT* x_base, KT* x_out, const PcgState* st) {
   hipStream_t s = c->stream;
   if (l == 0) {
     const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
-    hipLaunchKernelGGL(k_sell_resid<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s,
                        c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
-                       (const KT*)c->d_Ksell, x, b, r, c->nslices, c->nn, st);
+                       (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2,
+                       d_out, x_base, x_out, c->nslices, c->nn, st);
   } else {
-    const AmgLevel& L = A.lev[l];
-    hipLaunchKernelGGL(k_bsr6_resid<KT>, dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s, L.d_row_ptr,
-                       L.d_col, (const KT*)L.d_A, x, b, r, L.n, st);
+    const AmgLevel& L = c->amg.lev[l];
+    hipLaunchKernelGGL(k_bsr6_cheb<KT>, dim3(nblocks(L.n, 4)), dim3(256), 0, s, L.d_row_ptr,
+                       L.d_col, (const KT*)L.d_A, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1,
+                       (KT)c2, d_out, x_base, x_out, L.n, st);
   }
-  return ONSAS_OK;
 }
 
+// lambda_max(D^-1 A) of level l by power iteration (host-driven), from a hashed
+// start vector or from the last iterate
 template <typename KT>
-void smooth(Ctx* c, int l, const KT* v, KT* x, int accumulate, const PcgState* st) {
-  Amg& A = c->amg;
-  const AmgLevel& L = A.lev[l];
-  const KT om = (KT)L.omega;
-  if (l == 0)
-    hipLaunchKernelGGL((k_smooth<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, c->stream,
-                       (const KT*)c->d_Dinv, v, x, L.n, om, accumulate, st);
-  else
-    hipLaunchKernelGGL((k_smooth<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, c->stream,
-                       (const KT*)L.d_Dinv, v, x, L.n, om, accumulate, st);
-}
-
-// largest eigenvalue of D^-1 A on level l by power iteration (host-driven)
-template <typename KT>
-int estimate_omega(Ctx* c, int l) {
+int estimate_lambda(Ctx* c, int l, int iters, bool fresh) {
   Amg& A = c->amg;
   AmgLevel& L = A.lev[l];
   hipStream_t s = c->stream;
   const int64_t nd = (int64_t)L.bs * L.n;
-  KT* v = (KT*)(l == 0 ? c->d_pq : L.d_b);  // scratch: no PCG is running
-  KT* w = (KT*)(l == 0 ? c->d_pp1 : L.d_x);
-  KT* r = (KT*)L.d_r;
-  KT* zero = (KT*)(l == 0 ? c->d_pp0 : L.d_r);
-  hipLaunchKernelGGL(k_fill_hash<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, v, nd,
-                     l == 0 ? c->d_fixmask : nullptr, L.bs);
-  std::vector<double> h(1024);
-  double lam = 1.0;
-  for (int it = 0; it < 12; ++it) {
-    // w = D^-1 A v  (A v = 0 - (0 - A v): the residual kernel with b = 0 and x = -v)
-    hipLaunchKernelGGL(k_scale<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, v, nd, KT(-1));
-    if (l == 0) hipMemsetAsync(zero, 0, nd * sizeof(KT), s);
-    else hipMemsetAsync(zero, 0, nd * sizeof(KT), s);
-    ONSAS_TRY(apply_level_op<KT>(c, l, v, zero, r, nullptr));  // r = 0 - A(-v) = A v
-    hipLaunchKernelGGL(k_scale<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, v, nd, KT(-1));
-    const double om_save = L.omega;
-    L.omega = 1.0;
-    smooth<KT>(c, l, r, w, 0, nullptr);  // w = D^-1 A v
-    L.omega = om_save;
+  KT* v = (KT*)L.d_ev;
+  KT* w = (KT*)L.d_d1;
+  if (fresh)
+    hipLaunchKernelGGL(k_fill_hash<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, v, nd,
+                       l == 0 ? c->d_fixmask : nullptr, L.bs);
+  std::vector<double> h(512);
+  double lam = L.lam > 0 ? L.lam : 1.0;
+  for (int it = 0; it < iters; ++it) {
+    op_cheb<KT>(c, l, v, nullptr, nullptr, 0.0, 1.0, w, nullptr, nullptr, nullptr);  // w = -D^-1 A v
     hipLaunchKernelGGL(k_norm2<KT>, dim3(256), dim3(256), 0, s, w, nd, A.d_part);
     hipLaunchKernelGGL(k_norm2<KT>, dim3(256), dim3(256), 0, s, v, nd, A.d_part + 256);
     ONSAS_HIP_TRY(c, hipMemcpyAsync(h.data(), A.d_part, 512 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -654,14 +836,27 @@ int estimate_omega(Ctx* c, int l) {
     }
     if (!(nw > 0) || !(nv > 0)) break;
     lam = std::sqrt(nw / nv);
-    // v = w / |w|
-    ONSAS_HIP_TRY(c, hipMemcpyAsync(v, w, nd * sizeof(KT), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_scale<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, v, nd,
+    hipLaunchKernelGGL(k_scaled_copy<KT>, dim3(nblocks(nd, 256)), dim3(256), 0, s, w, v, nd,
                        (KT)(1.0 / std::sqrt(nw)));
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
-  L.omega = 1.0 / (1.1 * lam);
+  L.lam = lam;
   return ONSAS_OK;
+}
+
+template <typename KT, int BS>
+void launch_products(Ctx* c, AmgLevel& F, AmgLevel& C, const KT* Af, const KT* Df,
+                     const int32_t* colf) {
+  hipStream_t s = c->stream;
+  const double w = 4.0 / (3.0 * F.lam);
+  hipLaunchKernelGGL((k_sa_prolongator<KT, BS>), dim3(nblocks(6 * F.nnz_p, 256)), dim3(256), 0, s,
+                     F.d_p_row, F.d_p_col, F.d_agg, F.d_pk_ptr, F.d_pk, colf, Af, Df,
+                     (const KT*)F.d_T, w, (KT*)F.d_P, F.nnz_p);
+  hipLaunchKernelGGL((k_sa_ap<KT, BS>), dim3(nblocks(6 * F.nnz_w, 256)), dim3(256), 0, s,
+                     F.d_wl_ptr, F.d_wl_k, F.d_wl_p, Af, (const KT*)F.d_P, (KT*)F.d_W, F.nnz_w);
+  hipLaunchKernelGGL((k_sa_ptap<KT, BS>), dim3(nblocks(6 * C.nnzb, 256)), dim3(256), 0, s,
+                     F.d_gl_ptr, F.d_gl_p, F.d_gl_w, (const KT*)F.d_P, (const KT*)F.d_W,
+                     (KT*)C.d_A, C.nnzb);
 }
 
 template <typename KT>
@@ -669,20 +864,19 @@ int amg_numeric_t(Ctx* c) {
   Amg& A = c->amg;
   hipStream_t s = c->stream;
   const int nl = (int)A.lev.size();
-  for (int l = 1; l < nl; ++l) {
-    AmgLevel& L = A.lev[l];
-    const AmgLevel& F = A.lev[l - 1];
-    if (l == 1)
-      hipLaunchKernelGGL((k_galerkin<KT, 3>), dim3(nblocks(L.nnzb, 128)), dim3(128), 0, s, L.d_gptr,
-                         L.d_goff, L.d_gi, L.d_gj, (const KT*)c->d_Ksell, (int64_t)64,
-                         (const KT*)F.d_P, (KT*)L.d_A, L.nnzb);
+  ONSAS_TRY(ensure_bsr(c));  // level-0 values in block-CSR order
+  for (int l = 0; l + 1 < nl; ++l) {
+    AmgLevel& F = A.lev[l];
+    AmgLevel& C = A.lev[l + 1];
+    ONSAS_TRY(estimate_lambda<KT>(c, l, A.lam_set ? kPowerWarm : kPowerFirst, !A.lam_set));
+    if (l == 0)
+      launch_products<KT, 3>(c, F, C, (const KT*)c->d_K, (const KT*)c->d_Dinv, c->d_col);
     else
-      hipLaunchKernelGGL((k_galerkin<KT, 6>), dim3(nblocks(L.nnzb, 128)), dim3(128), 0, s, L.d_gptr,
-                         L.d_goff, L.d_gi, L.d_gj, (const KT*)F.d_A, (int64_t)1, (const KT*)F.d_P,
-                         (KT*)L.d_A, L.nnzb);
-    if (l < nl - 1)
-      hipLaunchKernelGGL(k_inv6<KT>, dim3(nblocks(L.n, 128)), dim3(128), 0, s, (KT*)L.d_A, L.d_diag,
-                         (KT*)L.d_Dinv, L.n);
+      launch_products<KT, 6>(c, F, C, (const KT*)F.d_A, (const KT*)F.d_Dinv, F.d_col);
+    if (l + 2 < nl)
+      hipLaunchKernelGGL(k_inv6<KT>, dim3(nblocks(C.n, 128)), dim3(128), 0, s, (KT*)C.d_A, C.d_diag,
+                         (KT*)C.d_Dinv, C.n);
+    ONSAS_HIP_TRY(c, hipGetLastError());
   }
   const AmgLevel& Lc = A.lev[nl - 1];
   hipLaunchKernelGGL(k_dense_scatter<KT>, dim3(nblocks(A.nc, 128)), dim3(128), 0, s, Lc.d_row_ptr,
@@ -691,52 +885,65 @@ int amg_numeric_t(Ctx* c) {
   hipLaunchKernelGGL(k_to_kt<KT>, dim3(nblocks((int64_t)A.nc * A.nc, 256)), dim3(256), 0, s,
                      A.d_dense, (KT*)A.d_Ainv, (int64_t)A.nc * A.nc);
   ONSAS_HIP_TRY(c, hipGetLastError());
-  if (!A.omega_set) {  // lambda_max(D^-1 A) varies little with the state
-    for (int l = 0; l < nl - 1; ++l) ONSAS_TRY(estimate_omega<KT>(c, l));
-    A.omega_set = true;
-  }
+  A.lam_set = true;
   A.stale = false;
   return ONSAS_OK;
 }
 
 template <typename KT>
-int vcycle_t(Ctx* c, const KT* b0, KT* x0, const PcgState* st) {
+int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st) {
   Amg& A = c->amg;
   hipStream_t s = c->stream;
   const int nl = (int)A.lev.size();
-  // down: pre-smooth, residual, restrict
-  for (int l = 0; l < nl - 1; ++l) {
+  {  // first Chebyshev step of the fine level; coarser ones come with the restriction
+    const AmgLevel& L = A.lev[0];
+    hipLaunchKernelGGL((k_jacobi<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
+                       (const KT*)c->d_Dinv, b0, (KT*)L.d_d0, L.n, (KT)cheb(L.lam).inv_theta, st);
+  }
+  // down: pre-smoothing, residual, restriction
+  for (int l = 0; l + 1 < nl; ++l) {
     AmgLevel& L = A.lev[l];
+    const Cheb ch = cheb(L.lam);
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
-    KT* x = l == 0 ? x0 : (KT*)L.d_x;
-    smooth<KT>(c, l, b, x, 0, st);
-    ONSAS_TRY(apply_level_op<KT>(c, l, x, b, (KT*)L.d_r, st));
-    const AmgLevel& C = A.lev[l + 1];
+    KT* x = (KT*)L.d_x;
+    KT* r = (KT*)L.d_r;
+    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st);
+    op_cheb<KT>(c, l, (const KT*)L.d_d1, r, r, 0.0, 0.0, nullptr, nullptr, nullptr, st);
+    AmgLevel& C = A.lev[l + 1];
+    const bool coarsest = l + 2 == nl;
+    const KT sc = coarsest ? KT(0) : (KT)cheb(C.lam).inv_theta;
+    KT* d0c = coarsest ? nullptr : (KT*)C.d_d0;
     if (l == 0)
-      hipLaunchKernelGGL((k_restrict<KT, 3>), dim3(nblocks(6 * C.n, 256)), dim3(256), 0, s, L.d_aptr,
-                         L.d_anodes, (const KT*)L.d_P, (const KT*)L.d_r, (KT*)C.d_b, C.n, st);
+      hipLaunchKernelGGL((k_sa_restrict<KT, 3>), dim3(nblocks(C.n, 4)), dim3(256), 0, s, L.d_rt_ptr,
+                         L.d_rt, L.d_p_row, (const KT*)L.d_P, (const KT*)r, (KT*)C.d_b,
+                         (const KT*)C.d_Dinv, sc, d0c, C.n, st);
     else
-      hipLaunchKernelGGL((k_restrict<KT, 6>), dim3(nblocks(6 * C.n, 256)), dim3(256), 0, s, L.d_aptr,
-                         L.d_anodes, (const KT*)L.d_P, (const KT*)L.d_r, (KT*)C.d_b, C.n, st);
+      hipLaunchKernelGGL((k_sa_restrict<KT, 6>), dim3(nblocks(C.n, 4)), dim3(256), 0, s, L.d_rt_ptr,
+                         L.d_rt, L.d_p_row, (const KT*)L.d_P, (const KT*)r, (KT*)C.d_b,
+                         (const KT*)C.d_Dinv, sc, d0c, C.n, st);
   }
   // coarsest: exact
   const AmgLevel& Lc = A.lev[nl - 1];
   hipLaunchKernelGGL(k_dense_apply<KT>, dim3(nblocks(A.nc, 64)), dim3(64), 0, s,
                      (const KT*)A.d_Ainv, (const KT*)Lc.d_b, (KT*)Lc.d_x, (int64_t)A.nc, st);
-  // up: prolong, post-smooth
+  // up: prolongation, post-smoothing of the residual
   for (int l = nl - 2; l >= 0; --l) {
     AmgLevel& L = A.lev[l];
-    const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
-    KT* x = l == 0 ? x0 : (KT*)L.d_x;
     const AmgLevel& C = A.lev[l + 1];
+    const Cheb ch = cheb(L.lam);
+    const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
+    KT* x = (KT*)L.d_x;
+    KT* r = (KT*)L.d_r;
+    KT* d0 = (KT*)L.d_d0;
+    KT* z = l == 0 ? z0 : x;
     if (l == 0)
-      hipLaunchKernelGGL((k_prolong<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s, L.d_agg,
-                         (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
+      hipLaunchKernelGGL((k_sa_prolong<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
+                         L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     else
-      hipLaunchKernelGGL((k_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s, L.d_agg,
-                         (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
-    ONSAS_TRY(apply_level_op<KT>(c, l, x, b, (KT*)L.d_r, st));
-    smooth<KT>(c, l, (const KT*)L.d_r, x, 1, st);
+      hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
+                         L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
+    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st);
+    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st);
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
@@ -757,9 +964,11 @@ int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st) {
 void amg_free(Ctx* c) {
   Amg& A = c->amg;
   for (AmgLevel& L : A.lev)
-    for (void* p : {(void*)L.d_row_ptr, (void*)L.d_col, (void*)L.d_diag, L.d_A, L.d_Dinv, L.d_b, L.d_x,
-                    L.d_r, (void*)L.d_agg, L.d_P, (void*)L.d_aptr, (void*)L.d_anodes,
-                    (void*)L.d_gptr, (void*)L.d_goff, (void*)L.d_gi, (void*)L.d_gj})
+    for (void* p : {(void*)L.d_row_ptr, (void*)L.d_col, (void*)L.d_diag, L.d_A, L.d_Dinv, L.d_b,
+                    L.d_x, L.d_r, L.d_d0, L.d_d1, L.d_ev, (void*)L.d_agg, L.d_T, (void*)L.d_p_ptr,
+                    (void*)L.d_p_col, (void*)L.d_p_row, L.d_P, (void*)L.d_pk_ptr, (void*)L.d_pk,
+                    (void*)L.d_rt_ptr, (void*)L.d_rt, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
+                    (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)A.d_dense, A.d_Ainv, (void*)A.d_part})
     if (p) (void)hipFree(p);

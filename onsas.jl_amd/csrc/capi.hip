@@ -355,6 +355,8 @@ int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s) {
     return set_err(c, ONSAS_ERR_ARG, "unknown linear solver kind");
   if (s->precond != ONSAS_PRECOND_BLOCK_JACOBI && s->precond != ONSAS_PRECOND_AMG)
     return set_err(c, ONSAS_ERR_ARG, "unknown preconditioner");
+  if (s->amg_refresh != ONSAS_AMG_REFRESH_STEP && s->amg_refresh != ONSAS_AMG_REFRESH_ASSEMBLY)
+    return set_err(c, ONSAS_ERR_ARG, "unknown multigrid refresh policy");
   c->lin = *s;
   if (c->lin.gmres_restart <= 0) c->lin.gmres_restart = 40;
   if (c->lin.check_every <= 0) c->lin.check_every = 16;
@@ -376,6 +378,7 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
                                     hipMemcpyHostToDevice, c->stream));
   ONSAS_TRY(onsas::launch_fext(c));
   ONSAS_TRY(onsas::reset_forcing(c));  // a new load step: no previous inner residual
+  c->amg.step_stale = true;            // ... and, by default, new multigrid coarse levels
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return ONSAS_OK;
 }

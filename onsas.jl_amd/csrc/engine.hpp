@@ -105,32 +105,46 @@ struct SellGroup {
   int64_t cbase;   // first contribution entry (entries of round q at cbase + 64 q + lane)
 };
 
-// Aggregation multigrid (amg.hip).  Level 0 is the fine tangent (its values
-// are the sliced-ELL copy and the block-Jacobi inverse of the context); levels
-// >= 1 have 6 dofs per node and 6x6 block-CSR values.
+// Smoothed-aggregation multigrid (amg.hip).  Level 0 is the fine tangent (its
+// values are the sliced-ELL copy, its block-CSR unpacking and the block-Jacobi
+// inverse of the context); levels >= 1 have 6 dofs per node and 6x6 block-CSR
+// values.  Each level but the coarsest owns the transfer to the next one: the
+// tentative prolongator T (bs x 6 per node), the smoothed prolongator
+// P = (I - w D^-1 A) T (bs x 6 blocks, block-CSR over fine rows), W = A P and
+// the fixed-order lists that form P, W and the next level's P^T W.
 struct AmgLevel {
   int bs = 6;                         // dofs per node
   int64_t n = 0;                      // nodes
-  std::vector<int32_t> row_ptr, col, diag;  // pattern (levels >= 1)
+  std::vector<int32_t> row_ptr, col;  // operator pattern (levels >= 1)
   int64_t nnzb = 0;
   int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_diag = nullptr;
-  void* d_A = nullptr;                // nnzb x 36 KT
-  void* d_Dinv = nullptr;             // n x 36 KT
-  double omega = 1.0;                 // smoother damping, 1 / (1.1 lambda_max(D^-1 A))
-  void *d_b = nullptr, *d_x = nullptr, *d_r = nullptr;  // cycle vectors (KT, bs n)
+  void* d_A = nullptr;                // nnzb x 36 KT (levels >= 1)
+  void* d_Dinv = nullptr;             // n x 36 KT (levels >= 1)
+  double lam = 0;                     // lambda_max(D^-1 A), power iteration
+  void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)
+  // cycle vectors (KT, bs n): right-hand side, iterate, residual, Chebyshev steps
+  void *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_d0 = nullptr, *d_d1 = nullptr;
   // transfer to the next coarser level
-  int32_t* d_agg = nullptr;           // node -> coarse node
-  void* d_P = nullptr;                // bs x 6 per node (KT)
-  int64_t* d_aptr = nullptr;          // coarse node -> member nodes (CSR)
-  int32_t* d_anodes = nullptr;
-  // Galerkin lists of this level's blocks (levels >= 1)
-  int64_t* d_gptr = nullptr;          // nnzb + 1
-  int64_t* d_goff = nullptr;          // value offset of the finer block
-  int32_t *d_gi = nullptr, *d_gj = nullptr;  // its finer row and column nodes
+  int32_t* d_agg = nullptr;           // node -> aggregate
+  void* d_T = nullptr;                // tentative prolongator, bs x 6 per node (KT)
+  int64_t nnz_p = 0;                  // blocks of P
+  int32_t *d_p_ptr = nullptr, *d_p_col = nullptr, *d_p_row = nullptr;
+  void* d_P = nullptr;                // nnz_p x bs x 6 (KT), row-major blocks
+  int64_t* d_pk_ptr = nullptr;        // per P block: the operator blocks of its row
+  int32_t* d_pk = nullptr;            //   whose column lies in its aggregate
+  int32_t *d_rt_ptr = nullptr, *d_rt = nullptr;  // per aggregate: its P blocks (P^T rows)
+  int64_t nnz_w = 0;                  // blocks of W = A P
+  void* d_W = nullptr;                // nnz_w x bs x 6 (KT)
+  int64_t* d_wl_ptr = nullptr;        // per W block: (operator block, P block) pairs
+  int32_t *d_wl_k = nullptr, *d_wl_p = nullptr;
+  int64_t* d_gl_ptr = nullptr;        // per block of the next level: (P block, W block)
+  int32_t *d_gl_p = nullptr, *d_gl_w = nullptr;  //   pairs of P^T W
 };
 
 struct Amg {
-  bool built = false, stale = true, omega_set = false;
+  bool built = false, lam_set = false;
+  bool stale = true;       // the coarse levels belong to an earlier tangent
+  bool step_stale = true;  // ... to an earlier load step
   std::vector<AmgLevel> lev;
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
   double* d_dense = nullptr;          // nc x nc FP64 work

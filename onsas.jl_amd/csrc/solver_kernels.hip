@@ -795,9 +795,14 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
   int chunk_iters = c->lin.check_every > 0 ? c->lin.check_every : 16;
   chunk_iters += chunk_iters & 1;  // even: the parity ping-pong repeats per chunk
   const bool amg = c->lin.precond == ONSAS_PRECOND_AMG;
-  if (amg) {  // hierarchy on first use, numerics after every assembly
+  if (amg) {  // hierarchy on first use; coarse numerics per the refresh policy
     if (!c->amg.built) ONSAS_TRY(amg_build(c));
-    if (c->amg.stale) ONSAS_TRY(amg_numeric(c));
+    const bool refresh = c->lin.amg_refresh == ONSAS_AMG_REFRESH_ASSEMBLY ? c->amg.stale
+                                                                          : c->amg.step_stale;
+    if (refresh || !c->amg.lam_set) {
+      ONSAS_TRY(amg_numeric(c));
+      c->amg.step_stale = false;
+    }
     hipLaunchKernelGGL(k_pcg_init_r<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
                        (KT*)c->d_px, (KT*)c->d_pr, part(c, kPartRR), part(c, kPartBB), np, c->chunk,
                        c->nn);

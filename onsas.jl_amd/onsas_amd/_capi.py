@@ -24,6 +24,7 @@ MAT_SVK, MAT_NEOHOOKEAN, MAT_LINEAR = 1, 2, 3
 STRAIN_ROTATED, STRAIN_GREEN = 0, 1
 LINSOLVE_AUTO, LINSOLVE_PCG, LINSOLVE_GMRES = 0, 1, 2
 PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1
+AMG_REFRESH_STEP, AMG_REFRESH_ASSEMBLY = 0, 1
 
 # every symbol declared in include/onsas_hip.h
 EXPORTED = (
@@ -42,7 +43,8 @@ class LinearSettings(C.Structure):
     _fields_ = [("kind", C.c_int32), ("maxiter", C.c_int32), ("rtol", C.c_double),
                 ("gmres_restart", C.c_int32), ("check_every", C.c_int32),
                 ("forcing", C.c_int32), ("newton_tol", C.c_double), ("eta_max", C.c_double),
-                ("eta_first", C.c_double), ("precond", C.c_int32)]
+                ("eta_first", C.c_double), ("precond", C.c_int32),
+                ("amg_refresh", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -225,10 +227,11 @@ class Context:
 
     def set_linear_solver(self, kind=LINSOLVE_AUTO, rtol=0.0, maxiter=0, gmres_restart=40,
                           check_every=16, forcing=False, newton_tol=0.0, eta_max=0.0,
-                          eta_first=0.0, precond=PRECOND_BLOCK_JACOBI):
+                          eta_first=0.0, precond=PRECOND_BLOCK_JACOBI,
+                          amg_refresh=AMG_REFRESH_STEP):
         s = LinearSettings(int(kind), int(maxiter), float(rtol), int(gmres_restart),
                            int(check_every), int(bool(forcing)), float(newton_tol),
-                           float(eta_max), float(eta_first), int(precond))
+                           float(eta_max), float(eta_first), int(precond), int(amg_refresh))
         self._check(self.L.onsas_set_linear_solver(self.h, C.byref(s)))
 
     # -- per step / iteration

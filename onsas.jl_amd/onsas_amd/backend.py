@@ -19,6 +19,7 @@ from .materials import IsotropicLinearElastic, NeoHookean, SVK
 
 _SOLVERS = {"auto": _capi.LINSOLVE_AUTO, "pcg": _capi.LINSOLVE_PCG, "gmres": _capi.LINSOLVE_GMRES}
 _PRECONDS = {"block-jacobi": _capi.PRECOND_BLOCK_JACOBI, "amg": _capi.PRECOND_AMG}
+_AMG_REFRESH = {"step": _capi.AMG_REFRESH_STEP, "assembly": _capi.AMG_REFRESH_ASSEMBLY}
 
 
 @dataclass
@@ -27,8 +28,10 @@ class MI355XBackend:
     or "fp64".  rtol/maxiter: inner-solve stopping rule (None: sqrt(eps) in fp64
     mode as the reference, 1e-6 in mixed mode; maxiter = free dofs).  solver:
     "auto" (PCG, GMRES if the tangent is indefinite), "pcg" or "gmres".
-    preconditioner: "block-jacobi" (3x3 nodal blocks) or "amg" (aggregation
-    multigrid V-cycle with rigid-body modes)."""
+    preconditioner: "block-jacobi" (3x3 nodal blocks) or "amg" (smoothed-
+    aggregation multigrid V-cycle with rigid-body modes and Chebyshev smoothing).
+    amg_refresh: "step" (coarse levels rebuilt once per load step) or "assembly"
+    (after every assembly); the finest level is always the current tangent."""
 
     precision: str = "mixed"
     rtol: float = None
@@ -43,6 +46,7 @@ class MI355XBackend:
     eta_max: float = 0.1
     eta_first: float = 1e-2
     preconditioner: str = "block-jacobi"
+    amg_refresh: str = "step"
 
     def precision_code(self):
         if self.precision not in ("mixed", "fp64"):
@@ -99,7 +103,8 @@ class GpuStaticState:
         b = self.backend
         self.ctx.set_linear_solver(_SOLVERS[b.solver], b.rtol or 0.0, b.maxiter or 0,
                                    b.gmres_restart, b.check_every, b.forcing, newton_tol,
-                                   b.eta_max, b.eta_first, _PRECONDS[b.preconditioner])
+                                   b.eta_max, b.eta_first, _PRECONDS[b.preconditioner],
+                                   _AMG_REFRESH[b.amg_refresh])
 
     # -- loads
     def apply_loads(self, t):

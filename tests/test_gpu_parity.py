@@ -473,8 +473,8 @@ def test_full_size_compression_cube_mixed():
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_amg_pcg_matches_direct(oracle, prec):
-    """PCG with the aggregation-multigrid V-cycle vs a sparse direct solve, and
-    fewer iterations than block-Jacobi on the same system."""
+    """PCG with the smoothed-aggregation V-cycle vs a sparse direct solve, and
+    far fewer iterations than block-Jacobi on the same system."""
     import scipy.sparse.linalg as spla
     print(f"[gpu] AMG PCG vs direct {prec}", flush=True)
     s, sm, p = workloads.cube_compression(n=8, jitter=0.05, seed=2)
@@ -505,16 +505,18 @@ def test_amg_pcg_matches_direct(oracle, prec):
             assert np.array_equal(x, x2) and it2 == it
         c.close()
     print(f"  iterations block-Jacobi {its[0]} AMG {its[1]}", flush=True)
-    assert its[_capi.PRECOND_AMG] < 0.5 * its[_capi.PRECOND_BLOCK_JACOBI]
+    assert its[_capi.PRECOND_AMG] < 0.2 * its[_capi.PRECOND_BLOCK_JACOBI]
 
 
+@pytest.mark.parametrize("refresh", ["step", "assembly"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_newton_with_amg(prec):
-    """Compression cube with the AMG preconditioner: the closed-form state."""
-    print(f"[gpu] Newton with AMG {prec}", flush=True)
+def test_newton_with_amg(prec, refresh):
+    """Compression cube with the AMG preconditioner, its coarse levels rebuilt
+    once per load step or after every assembly: the closed-form state."""
+    print(f"[gpu] Newton with AMG {prec} refresh {refresh}", flush=True)
     s, sm, p = workloads.cube_compression(n=6, jitter=0.05, seed=2)
     nsteps = 4
-    sol = _newton(s, nsteps, prec, preconditioner="amg")
+    sol = _newton(s, nsteps, prec, preconditioner="amg", amg_refresh=refresh)
     for k, t in enumerate(np.linspace(1 / nsteps, 1.0, nsteps)):
         a = workloads.nh_uniaxial_alpha(-p * t, 8.3e5, 3.8e5)
         b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
