@@ -58,6 +58,9 @@ typedef struct onsas_ctx onsas_ctx;
 #define ONSAS_AMG_REFRESH_STEP 0     /* multigrid coarse levels rebuilt at the first solve of
                                         each load step (after onsas_set_loads)             */
 #define ONSAS_AMG_REFRESH_ASSEMBLY 1 /* ... rebuilt after every assembly                   */
+#define ONSAS_AMG_REFRESH_ONCE 2     /* ... built from the first solve's tangent only (with
+                                        the smoothing bounds; the finest level is still the
+                                        current tangent)                                    */
 
 /* Linear-solver settings; replace the tolerances of
  * src/StructuralSolvers/StructuralSolvers.jl:229-234 (abstol 0, reltol sqrt(eps),

@@ -41,6 +41,9 @@ constexpr int kMaxCoarseNodes = 12;  // coarsest level: <= 72 dofs, inverted in 
 constexpr double kChebRatio = 30.0;  // smoothing interval [hi / 30, hi], hi = 1.1 lambda
 constexpr int kPowerFirst = 15;      // power-iteration steps, first estimate
 constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
+constexpr int64_t kSell6MinRows = 2048;  // smaller levels: one wave per block row
+constexpr int kSell6Window = 256;     // rows sorted by length within windows of this size
+constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 4 waves restrict
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -82,14 +85,15 @@ Cheb cheb(double lam) {
 
 // P block p = (i, J), one thread per column c: T_i[:, c] if J is the aggregate of
 // i, minus w D_i^-1 times the sum over the operator blocks k of row i whose column
-// j lies in J (list pk, row order) of A_k T_j[:, c].  FP64 sums.
+// j lies in J (list pk, row order) of A_k T_j[:, c].  FP64 sums.  Written to P
+// (row order) and to PT at rt_pos[p] (the restriction's order).
 template <typename KT, int BS>
 __global__ __launch_bounds__(256) void k_sa_prolongator(
     const int32_t* __restrict__ p_row, const int32_t* __restrict__ p_col,
     const int32_t* __restrict__ agg, const int64_t* __restrict__ pk_ptr,
     const int32_t* __restrict__ pk, const int32_t* __restrict__ col, const KT* __restrict__ A,
     const KT* __restrict__ Dinv, const KT* __restrict__ T, double w, KT* __restrict__ P,
-    int64_t nnz_p) {
+    const int32_t* __restrict__ rt_pos, KT* __restrict__ PT, int64_t nnz_p) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * nnz_p) return;
   const int64_t p = t / 6;
@@ -114,12 +118,15 @@ __global__ __launch_bounds__(256) void k_sa_prolongator(
   const bool own = p_col[p] == agg[i];
   const KT* ti = T + (int64_t)BS * 6 * i + c;
   KT* o = P + (int64_t)BS * 6 * p + c;
+  KT* ot = PT + (int64_t)BS * 6 * rt_pos[p] + c;
 #pragma unroll
   for (int r = 0; r < BS; ++r) {
     double ds = 0;
 #pragma unroll
     for (int m = 0; m < BS; ++m) ds += (double)d[BS * r + m] * s[m];
-    o[6 * r] = (KT)((own ? (double)ti[6 * r] : 0.0) - w * ds);
+    const KT v = (KT)((own ? (double)ti[6 * r] : 0.0) - w * ds);
+    o[6 * r] = v;
+    ot[6 * r] = v;
   }
 }
 
@@ -334,7 +341,7 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
   }
 }
 
-// Levels >= 1 (6x6 block-CSR): one wave per block row, lanes over its blocks,
+// Small levels (6x6 block-CSR): one wave per block row, lanes over its blocks,
 // lanes 0..5 finish one component each.
 template <typename KT>
 __global__ __launch_bounds__(256) void k_bsr6_cheb(
@@ -375,6 +382,75 @@ __global__ __launch_bounds__(256) void k_bsr6_cheb(
   if (x_out) x_out[6 * I + l] = (x_base ? x_base[6 * I + l] : KT(0)) + vu + d;
 }
 
+// Large levels >= 1 on their sliced-ELL copy: one wave per slice of 8 rows, 8
+// lanes per row; every value load of a round reads 64 x 16 contiguous bytes.
+// After the reduction over the 8 lanes of a row, lanes ph < 6 finish component
+// ph of row s6_row[8 s + r] (rows sorted by length within windows; -1: none).
+template <typename KT>
+__global__ __launch_bounds__(256) void k_sell6_cheb(
+    const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
+    const int32_t* __restrict__ s6_row, const KT* __restrict__ A6, const KT* __restrict__ v,
+    const KT* b, KT* r_out,
+    const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
+    KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
+  if (pcg_stopped(st)) return;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (s >= nslices) return;
+  KT acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int32_t q = s6_ptr[s]; q < s6_ptr[s + 1]; ++q) {
+    const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)s6_col[64 * (int64_t)q + l], 8);
+    KT xv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xv[c] = xj[c];
+    const KT* a = A6 + 36 * 64 * (int64_t)q + 4 * l;
+#pragma unroll
+    for (int ch = 0; ch < 9; ++ch) {
+      const KT* ap = (const KT*)__builtin_assume_aligned(a + 256 * ch, 16);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = 4 * ch + j;
+        acc[e / 6] += ap[j] * xv[e % 6];
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1)
+#pragma unroll
+    for (int u = 0; u < 6; ++u) acc[u] += __shfl_xor(acc[u], o, 64);
+  const int64_t I = s6_row[8 * s + (l & 7)];
+  const int u = l >> 3;
+  if (u >= 6 || I < 0) return;
+  KT r[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) r[m] = (b ? b[6 * I + m] : KT(0)) - acc[m];
+  const KT ru = pick6(r, u);
+  if (r_out) r_out[6 * I + u] = ru;
+  if (!d_out && !x_out) return;
+  const KT* dm = Dinv + 36 * I + 6 * u;
+  KT dr = 0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) dr += dm[m] * r[m];
+  const KT vu = v[6 * I + u];
+  const KT d = c1 * vu + c2 * dr;
+  if (d_out) d_out[6 * I + u] = d;
+  if (x_out) x_out[6 * I + u] = (x_base ? x_base[6 * I + u] : KT(0)) + vu + d;
+}
+
+// Block-CSR -> sliced-ELL values: one thread per (round slot, value).
+template <typename KT>
+__global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restrict__ A,
+                           KT* __restrict__ A6, int64_t nslots) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 36 * nslots) return;
+  const int64_t slot = t / 36;  // 64 q + l
+  const int e = (int)(t % 36);
+  const int64_t q = slot >> 6;
+  const int l = (int)(slot & 63);
+  const int32_t k = s6_src[slot];
+  A6[((9 * q + e / 4) * 64 + l) * 4 + (e & 3)] = k >= 0 ? A[36 * (int64_t)k + e] : KT(0);
+}
+
 // y = s D^-1 v, one thread per (node, component).
 template <typename KT, int BS>
 __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, KT* __restrict__ y,
@@ -392,24 +468,27 @@ __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
   y[t] = s * acc;
 }
 
-// b_c[I] = sum over the P blocks p of column I (list rt, row order) of
-// P_p^T r_row(p): a wave per aggregate, lanes over its blocks.  With d0 non-null
-// also d0[I] = s D_c^-1 b_c[I] (the first Chebyshev step of the coarse level).
-template <typename KT, int BS>
+// b_c[I] = sum over the P blocks of column I (entries rt_ptr[I].., row order; PT
+// holds them contiguously) of P_e^T r_row(e): a wave per aggregate, lanes over its
+// entries.  With d0 non-null also d0[I] = s D_c^-1 b_c[I] (the first Chebyshev
+// step of the coarse level).
+// WPA waves per aggregate (1 or 4; with 4 the workgroup's waves add their sums
+// in wave order through LDS).
+template <typename KT, int BS, int WPA>
 __global__ __launch_bounds__(256) void k_sa_restrict(
-    const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt,
-    const int32_t* __restrict__ p_row, const KT* __restrict__ P, const KT* __restrict__ rf,
-    KT* __restrict__ bc, const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc,
-    const PcgState* st) {
+    const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_row,
+    const KT* __restrict__ PT, const KT* __restrict__ rf, KT* __restrict__ bc,
+    const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc, const PcgState* st) {
+  static_assert(WPA == 1 || WPA == 4, "one or four waves per aggregate");
+  __shared__ KT sh[4][6];
   if (pcg_stopped(st)) return;
-  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int l = threadIdx.x & 63;
-  if (I >= nc) return;
+  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / (64 * WPA);
+  const int l = threadIdx.x % (64 * WPA);
+  if (WPA == 1 && I >= nc) return;  // WPA 4: one aggregate per workgroup, I < nc
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t e = rt_ptr[I] + l; e < rt_ptr[I + 1]; e += 64) {
-    const int32_t p = rt[e];
-    const int64_t i = p_row[p];
-    const KT* pp = P + (int64_t)BS * 6 * p;
+  for (int32_t e = rt_ptr[I] + l; e < rt_ptr[I + 1]; e += 64 * WPA) {
+    const int64_t i = rt_row[e];
+    const KT* pp = (const KT*)__builtin_assume_aligned(PT + (int64_t)BS * 6 * e, 8);
     KT rv[BS];
 #pragma unroll
     for (int a = 0; a < BS; ++a) rv[a] = rf[BS * i + a];
@@ -419,6 +498,14 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
       for (int u = 0; u < 6; ++u) acc[u] += pp[6 * a + u] * rv[a];
   }
   wave_sum6(acc);
+  if (WPA == 4) {
+    if ((threadIdx.x & 63) < 6) sh[threadIdx.x >> 6][threadIdx.x & 63] = pick6(acc, threadIdx.x & 63);
+    __syncthreads();
+    if (l < 6) {
+#pragma unroll
+      for (int u = 0; u < 6; ++u) acc[u] = sh[0][u] + sh[1][u] + sh[2][u] + sh[3][u];
+    }
+  }
   if (l >= 6) return;
   bc[6 * I + l] = pick6(acc, l);
   if (d0) {
@@ -443,8 +530,8 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
   const int a = (int)(t % BS);
   KT acc = 0;
   for (int32_t p = p_ptr[i]; p < p_ptr[i + 1]; ++p) {
-    const KT* pp = P + (int64_t)BS * 6 * p + 6 * a;
-    const KT* xj = xc + 6 * (int64_t)p_col[p];
+    const KT* pp = (const KT*)__builtin_assume_aligned(P + (int64_t)BS * 6 * p + 6 * a, 8);
+    const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)p_col[p], 8);
 #pragma unroll
     for (int u = 0; u < 6; ++u) acc += pp[u] * xj[u];
   }
@@ -706,12 +793,17 @@ int amg_build(Ctx* c) {
     }
     const int64_t nnz_w = (int64_t)w_col.size();
     // P^T: per aggregate its P blocks, in row order
-    std::vector<int32_t> rt_ptr(na + 1, 0), rt(nnz_p);
+    std::vector<int32_t> rt_ptr(na + 1, 0), rt(nnz_p), rt_row(nnz_p), rt_pos(nnz_p);
     for (int64_t p = 0; p < nnz_p; ++p) rt_ptr[p_col[p] + 1]++;
     for (int64_t a = 0; a < na; ++a) rt_ptr[a + 1] += rt_ptr[a];
     {
       std::vector<int32_t> pos(rt_ptr.begin(), rt_ptr.end() - 1);
-      for (int64_t p = 0; p < nnz_p; ++p) rt[pos[p_col[p]]++] = (int32_t)p;
+      for (int64_t p = 0; p < nnz_p; ++p) {
+        const int32_t e = pos[p_col[p]]++;
+        rt[e] = (int32_t)p;
+        rt_row[e] = p_row[p];
+        rt_pos[p] = e;
+      }
     }
     // next level: pattern of P^T W and its lists of (P block, W block) pairs
     AmgLevel Lc;
@@ -735,11 +827,43 @@ int amg_build(Ctx* c) {
         return set_err(c, ONSAS_ERR_UNSUPPORTED, "multigrid: coarse node without a diagonal block");
     }
     Lc.nnzb = (int64_t)Lc.col.size();
+    // sliced-ELL layout of a large coarse operator (not the coarsest level):
+    // rows ordered by decreasing length within windows, 8 per slice
+    std::vector<int32_t> s6_ptr(1, 0), s6_col, s6_src, s6_row;
+    if (na > kMaxCoarseNodes && na >= kSell6MinRows) {
+      std::vector<int32_t> order(na);
+      for (int64_t I = 0; I < na; ++I) order[I] = (int32_t)I;
+      auto len = [&](int32_t I) { return Lc.row_ptr[I + 1] - Lc.row_ptr[I]; };
+      for (int64_t w0 = 0; w0 < na; w0 += kSell6Window)
+        std::stable_sort(order.begin() + w0, order.begin() + std::min<int64_t>(w0 + kSell6Window, na),
+                         [&](int32_t x, int32_t y) { return len(x) > len(y); });
+      for (int64_t r0 = 0; r0 < na; r0 += 8) {
+        int32_t width = 0;
+        for (int r = 0; r < 8; ++r) {
+          const int32_t I = r0 + r < na ? order[r0 + r] : -1;
+          s6_row.push_back(I);
+          if (I >= 0) width = std::max(width, (len(I) + 7) / 8);
+        }
+        for (int32_t m = 0; m < width; ++m)
+          for (int l = 0; l < 64; ++l) {
+            const int32_t I = s6_row[r0 + (l & 7)];
+            const int32_t slot = 8 * m + (l >> 3);
+            const bool ok = I >= 0 && slot < len(I);
+            const int32_t k = ok ? Lc.row_ptr[I] + slot : -1;
+            s6_src.push_back(k);
+            s6_col.push_back(ok ? Lc.col[k] : 0);
+          }
+        s6_ptr.push_back(s6_ptr.back() + width);
+      }
+      Lc.nslices6 = (int64_t)s6_ptr.size() - 1;
+      Lc.nrounds6 = s6_ptr.back();
+    }
 
     // device arrays of the transfer (finer level) and of the coarse level
     AmgLevel& F = A.lev.back();
     F.nnz_p = nnz_p;
     F.nnz_w = nnz_w;
+    F.restrict_wide = nnz_p > (int64_t)kRestrictWide * na;
     ONSAS_TRY(upload(c, &F.d_agg, agg));
     ONSAS_TRY(f64 ? upload_kt<double>(c, &F.d_T, T) : upload_kt<float>(c, &F.d_T, T));
     ONSAS_TRY(upload(c, &F.d_p_ptr, p_ptr));
@@ -748,7 +872,8 @@ int amg_build(Ctx* c) {
     ONSAS_TRY(upload(c, &F.d_pk_ptr, pk_ptr));
     ONSAS_TRY(upload(c, &F.d_pk, pk));
     ONSAS_TRY(upload(c, &F.d_rt_ptr, rt_ptr));
-    ONSAS_TRY(upload(c, &F.d_rt, rt));
+    ONSAS_TRY(upload(c, &F.d_rt_row, rt_row));
+    ONSAS_TRY(upload(c, &F.d_rt_pos, rt_pos));
     ONSAS_TRY(upload(c, &F.d_wl_ptr, wl_ptr));
     ONSAS_TRY(upload(c, &F.d_wl_k, wl_k));
     ONSAS_TRY(upload(c, &F.d_wl_p, wl_p));
@@ -756,11 +881,19 @@ int amg_build(Ctx* c) {
     ONSAS_TRY(upload(c, &F.d_gl_p, gl_p));
     ONSAS_TRY(upload(c, &F.d_gl_w, gl_w));
     ONSAS_TRY(alloc(c, &F.d_P, c->kt * bs * 6 * (size_t)nnz_p));
+    ONSAS_TRY(alloc(c, &F.d_PT, c->kt * bs * 6 * (size_t)nnz_p));
     ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)nnz_w));
     ONSAS_TRY(upload(c, &Lc.d_row_ptr, Lc.row_ptr));
     ONSAS_TRY(upload(c, &Lc.d_col, Lc.col));
     ONSAS_TRY(upload(c, &Lc.d_diag, diag));
     ONSAS_TRY(alloc(c, &Lc.d_A, c->kt * 36 * (size_t)Lc.nnzb));
+    if (Lc.nslices6 > 0) {
+      ONSAS_TRY(upload(c, &Lc.d_s6_ptr, s6_ptr));
+      ONSAS_TRY(upload(c, &Lc.d_s6_col, s6_col));
+      ONSAS_TRY(upload(c, &Lc.d_s6_src, s6_src));
+      ONSAS_TRY(upload(c, &Lc.d_s6_row, s6_row));
+      ONSAS_TRY(alloc(c, &Lc.d_A6, c->kt * 36 * 64 * (size_t)Lc.nrounds6));
+    }
     ONSAS_TRY(alloc(c, &Lc.d_Dinv, c->kt * 36 * (size_t)na));
     for (void** v : {&Lc.d_b, &Lc.d_x, &Lc.d_r, &Lc.d_d0, &Lc.d_d1, &Lc.d_ev})
       ONSAS_TRY(alloc(c, v, c->kt * 6 * (size_t)na));
@@ -802,9 +935,15 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
                        d_out, x_base, x_out, c->nslices, c->nn, st);
   } else {
     const AmgLevel& L = c->amg.lev[l];
-    hipLaunchKernelGGL(k_bsr6_cheb<KT>, dim3(nblocks(L.n, 4)), dim3(256), 0, s, L.d_row_ptr,
-                       L.d_col, (const KT*)L.d_A, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1,
-                       (KT)c2, d_out, x_base, x_out, L.n, st);
+    if (L.nslices6 > 0)
+      hipLaunchKernelGGL(k_sell6_cheb<KT>, dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
+                         L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const KT*)L.d_A6, v, b, r_out,
+                         (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6,
+                         L.n, st);
+    else
+      hipLaunchKernelGGL(k_bsr6_cheb<KT>, dim3(nblocks(L.n, 4)), dim3(256), 0, s, L.d_row_ptr,
+                         L.d_col, (const KT*)L.d_A, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1,
+                         (KT)c2, d_out, x_base, x_out, L.n, st);
   }
 }
 
@@ -851,7 +990,7 @@ void launch_products(Ctx* c, AmgLevel& F, AmgLevel& C, const KT* Af, const KT* D
   const double w = 4.0 / (3.0 * F.lam);
   hipLaunchKernelGGL((k_sa_prolongator<KT, BS>), dim3(nblocks(6 * F.nnz_p, 256)), dim3(256), 0, s,
                      F.d_p_row, F.d_p_col, F.d_agg, F.d_pk_ptr, F.d_pk, colf, Af, Df,
-                     (const KT*)F.d_T, w, (KT*)F.d_P, F.nnz_p);
+                     (const KT*)F.d_T, w, (KT*)F.d_P, F.d_rt_pos, (KT*)F.d_PT, F.nnz_p);
   hipLaunchKernelGGL((k_sa_ap<KT, BS>), dim3(nblocks(6 * F.nnz_w, 256)), dim3(256), 0, s,
                      F.d_wl_ptr, F.d_wl_k, F.d_wl_p, Af, (const KT*)F.d_P, (KT*)F.d_W, F.nnz_w);
   hipLaunchKernelGGL((k_sa_ptap<KT, BS>), dim3(nblocks(6 * C.nnzb, 256)), dim3(256), 0, s,
@@ -873,9 +1012,13 @@ int amg_numeric_t(Ctx* c) {
       launch_products<KT, 3>(c, F, C, (const KT*)c->d_K, (const KT*)c->d_Dinv, c->d_col);
     else
       launch_products<KT, 6>(c, F, C, (const KT*)F.d_A, (const KT*)F.d_Dinv, F.d_col);
-    if (l + 2 < nl)
+    if (l + 2 < nl) {
       hipLaunchKernelGGL(k_inv6<KT>, dim3(nblocks(C.n, 128)), dim3(128), 0, s, (KT*)C.d_A, C.d_diag,
                          (KT*)C.d_Dinv, C.n);
+      if (C.nslices6 > 0)
+        hipLaunchKernelGGL(k_to_sell6<KT>, dim3(nblocks(36 * 64 * C.nrounds6, 256)), dim3(256), 0,
+                           s, C.d_s6_src, (const KT*)C.d_A, (KT*)C.d_A6, 64 * C.nrounds6);
+    }
     ONSAS_HIP_TRY(c, hipGetLastError());
   }
   const AmgLevel& Lc = A.lev[nl - 1];
@@ -890,12 +1033,26 @@ int amg_numeric_t(Ctx* c) {
   return ONSAS_OK;
 }
 
+template <typename KT, int BS>
+void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, KT sc, KT* d0c,
+                     const PcgState* st) {
+  if (L.restrict_wide)
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 4>), dim3((unsigned)C.n), dim3(256), 0, c->stream,
+                       L.d_rt_ptr, L.d_rt_row, (const KT*)L.d_PT, r, (KT*)C.d_b,
+                       (const KT*)C.d_Dinv, sc, d0c, C.n, st);
+  else
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 1>), dim3(nblocks(C.n, 4)), dim3(256), 0, c->stream,
+                       L.d_rt_ptr, L.d_rt_row, (const KT*)L.d_PT, r, (KT*)C.d_b,
+                       (const KT*)C.d_Dinv, sc, d0c, C.n, st);
+}
+
 template <typename KT>
-int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st) {
+int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready) {
   Amg& A = c->amg;
   hipStream_t s = c->stream;
   const int nl = (int)A.lev.size();
-  {  // first Chebyshev step of the fine level; coarser ones come with the restriction
+  if (!d0_ready) {  // first Chebyshev step of the fine level; coarser ones come with
+                    // the restriction
     const AmgLevel& L = A.lev[0];
     hipLaunchKernelGGL((k_jacobi<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
                        (const KT*)c->d_Dinv, b0, (KT*)L.d_d0, L.n, (KT)cheb(L.lam).inv_theta, st);
@@ -914,13 +1071,9 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st) {
     const KT sc = coarsest ? KT(0) : (KT)cheb(C.lam).inv_theta;
     KT* d0c = coarsest ? nullptr : (KT*)C.d_d0;
     if (l == 0)
-      hipLaunchKernelGGL((k_sa_restrict<KT, 3>), dim3(nblocks(C.n, 4)), dim3(256), 0, s, L.d_rt_ptr,
-                         L.d_rt, L.d_p_row, (const KT*)L.d_P, (const KT*)r, (KT*)C.d_b,
-                         (const KT*)C.d_Dinv, sc, d0c, C.n, st);
+      launch_restrict<KT, 3>(c, L, C, r, sc, d0c, st);
     else
-      hipLaunchKernelGGL((k_sa_restrict<KT, 6>), dim3(nblocks(C.n, 4)), dim3(256), 0, s, L.d_rt_ptr,
-                         L.d_rt, L.d_p_row, (const KT*)L.d_P, (const KT*)r, (KT*)C.d_b,
-                         (const KT*)C.d_Dinv, sc, d0c, C.n, st);
+      launch_restrict<KT, 6>(c, L, C, r, sc, d0c, st);
   }
   // coarsest: exact
   const AmgLevel& Lc = A.lev[nl - 1];
@@ -955,10 +1108,15 @@ int amg_numeric(Ctx* c) {
   return c->precision == ONSAS_PRECISION_FP64 ? amg_numeric_t<double>(c) : amg_numeric_t<float>(c);
 }
 
-int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st) {
+int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready) {
   return c->precision == ONSAS_PRECISION_FP64
-             ? vcycle_t<double>(c, (const double*)b, (double*)x, st)
-             : vcycle_t<float>(c, (const float*)b, (float*)x, st);
+             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready)
+             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready);
+}
+
+double amg_fine_smoother(Ctx* c, void** d0) {
+  *d0 = c->amg.lev[0].d_d0;
+  return cheb(c->amg.lev[0].lam).inv_theta;
 }
 
 void amg_free(Ctx* c) {
@@ -967,8 +1125,10 @@ void amg_free(Ctx* c) {
     for (void* p : {(void*)L.d_row_ptr, (void*)L.d_col, (void*)L.d_diag, L.d_A, L.d_Dinv, L.d_b,
                     L.d_x, L.d_r, L.d_d0, L.d_d1, L.d_ev, (void*)L.d_agg, L.d_T, (void*)L.d_p_ptr,
                     (void*)L.d_p_col, (void*)L.d_p_row, L.d_P, (void*)L.d_pk_ptr, (void*)L.d_pk,
-                    (void*)L.d_rt_ptr, (void*)L.d_rt, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
-                    (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w})
+                    (void*)L.d_rt_ptr, (void*)L.d_rt_row, (void*)L.d_rt_pos, L.d_PT, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
+                    (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w,
+                    (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
+                    (void*)L.d_s6_row, L.d_A6})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)A.d_dense, A.d_Ainv, (void*)A.d_part})
     if (p) (void)hipFree(p);

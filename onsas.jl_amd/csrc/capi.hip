@@ -355,7 +355,8 @@ int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s) {
     return set_err(c, ONSAS_ERR_ARG, "unknown linear solver kind");
   if (s->precond != ONSAS_PRECOND_BLOCK_JACOBI && s->precond != ONSAS_PRECOND_AMG)
     return set_err(c, ONSAS_ERR_ARG, "unknown preconditioner");
-  if (s->amg_refresh != ONSAS_AMG_REFRESH_STEP && s->amg_refresh != ONSAS_AMG_REFRESH_ASSEMBLY)
+  if (s->amg_refresh != ONSAS_AMG_REFRESH_STEP && s->amg_refresh != ONSAS_AMG_REFRESH_ASSEMBLY &&
+      s->amg_refresh != ONSAS_AMG_REFRESH_ONCE)
     return set_err(c, ONSAS_ERR_ARG, "unknown multigrid refresh policy");
   c->lin = *s;
   if (c->lin.gmres_restart <= 0) c->lin.gmres_restart = 40;

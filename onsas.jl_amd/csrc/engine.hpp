@@ -120,6 +120,16 @@ struct AmgLevel {
   int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_diag = nullptr;
   void* d_A = nullptr;                // nnzb x 36 KT (levels >= 1)
   void* d_Dinv = nullptr;             // n x 36 KT (levels >= 1)
+  // sliced-ELL copy of A for the cycle (large levels >= 1 but the coarsest):
+  // rows ordered by decreasing length within windows, slices of 8 of them, one
+  // wave each; lane l = 8 ph + r serves row s6_row[8 s + r] (-1: none), slot
+  // 8 m + ph of round m.  Round q of the level (s6_ptr[s] <= q < s6_ptr[s+1])
+  // holds lane l's block column at s6_col[64 q + l] and its 36 values in 9
+  // chunks of 4 at A6[((9 q + chunk) 64 + l) 4 + j]; s6_src is the source block
+  // in A (-1: padding, zero values).
+  int64_t nslices6 = 0, nrounds6 = 0;
+  int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
+  void* d_A6 = nullptr;
   double lam = 0;                     // lambda_max(D^-1 A), power iteration
   void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)
   // cycle vectors (KT, bs n): right-hand side, iterate, residual, Chebyshev steps
@@ -128,11 +138,15 @@ struct AmgLevel {
   int32_t* d_agg = nullptr;           // node -> aggregate
   void* d_T = nullptr;                // tentative prolongator, bs x 6 per node (KT)
   int64_t nnz_p = 0;                  // blocks of P
+  bool restrict_wide = false;         // restriction with 4 waves per aggregate
   int32_t *d_p_ptr = nullptr, *d_p_col = nullptr, *d_p_row = nullptr;
   void* d_P = nullptr;                // nnz_p x bs x 6 (KT), row-major blocks
   int64_t* d_pk_ptr = nullptr;        // per P block: the operator blocks of its row
   int32_t* d_pk = nullptr;            //   whose column lies in its aggregate
-  int32_t *d_rt_ptr = nullptr, *d_rt = nullptr;  // per aggregate: its P blocks (P^T rows)
+  // P^T rows: per aggregate its P blocks in row order (rt_ptr), each entry's
+  // fine row (rt_row) and a copy of P in that order (PT); rt_pos: P block -> entry
+  int32_t *d_rt_ptr = nullptr, *d_rt_row = nullptr, *d_rt_pos = nullptr;
+  void* d_PT = nullptr;
   int64_t nnz_w = 0;                  // blocks of W = A P
   void* d_W = nullptr;                // nnz_w x bs x 6 (KT)
   int64_t* d_wl_ptr = nullptr;        // per W block: (operator block, P block) pairs
@@ -303,7 +317,10 @@ int launch_locate(Ctx* c, int g, const double* px, const double* py, const doubl
 // ---- amg.hip ----
 int amg_build(Ctx* c);                                          // hierarchy (host) + arrays
 int amg_numeric(Ctx* c);                                        // Galerkin, inverses, omega
-int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st);  // x = M^-1 b
+// x = M^-1 b; with d0_ready the first fine-level smoother step is already in
+// the buffer amg_fine_smoother names (fused into the caller's kernel)
+int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready);
+double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
 void amg_free(Ctx* c);
 
 // ---- solver_kernels.hip ----

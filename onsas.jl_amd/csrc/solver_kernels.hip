@@ -493,14 +493,18 @@ __global__ __launch_bounds__(NT) void k_pcg_init_r(const double* __restrict__ b,
   }
 }
 
-// alpha = r.z / p.q; x += alpha p; r -= alpha q; partials r.r; breakdown test
+// alpha = r.z / p.q; x += alpha p; r -= alpha q; partials r.r; breakdown test.
+// With d0 non-null also d0 = s D^-1 r (3x3 nodal blocks): the first step of the
+// multigrid smoother.
 template <typename KT>
 __global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, const KT* __restrict__ q,
                                                      KT* __restrict__ x, KT* __restrict__ r,
                                                      const double* __restrict__ prz_cur,
                                                      const double* __restrict__ ppq,
                                                      double* __restrict__ prr, PcgState* __restrict__ st,
-                                                     int np, int64_t chunk, int64_t nn) {
+                                                     const KT* __restrict__ Dinv, KT s,
+                                                     KT* __restrict__ d0, int np, int64_t chunk,
+                                                     int64_t nn) {
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
   double rz, pq, unused;
@@ -517,12 +521,19 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, con
   wg_rows(np, chunk, nn, r0, r1);
   double rrn = 0;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
+    KT ri[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       x[3 * i + d] += alpha * p[3 * i + d];
-      const KT ri = r[3 * i + d] - alpha * q[3 * i + d];
-      r[3 * i + d] = ri;
-      rrn += (double)ri * (double)ri;
+      ri[d] = r[3 * i + d] - alpha * q[3 * i + d];
+      r[3 * i + d] = ri[d];
+      rrn += (double)ri[d] * (double)ri[d];
+    }
+    if (d0) {
+      const KT* dm = Dinv + 9 * i;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        d0[3 * i + d] = s * (dm[3 * d] * ri[0] + dm[3 * d + 1] * ri[1] + dm[3 * d + 2] * ri[2]);
     }
   }
   rrn = block_sum(rrn, sh);
@@ -750,10 +761,12 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
                        RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np,
                        c->chunk, c->nn);
   if (c->lin.precond == ONSAS_PRECOND_AMG) {
+    KT* d0 = nullptr;
+    const double s0 = amg_fine_smoother(c, (void**)&d0);
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
-                       c->np, c->chunk, c->nn);
-    (void)launch_vcycle(c, c->d_pr, c->d_pz, st);
+                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, c->chunk, c->nn);
+    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true);
     hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
                        (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
     return;
@@ -797,8 +810,10 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
   const bool amg = c->lin.precond == ONSAS_PRECOND_AMG;
   if (amg) {  // hierarchy on first use; coarse numerics per the refresh policy
     if (!c->amg.built) ONSAS_TRY(amg_build(c));
-    const bool refresh = c->lin.amg_refresh == ONSAS_AMG_REFRESH_ASSEMBLY ? c->amg.stale
-                                                                          : c->amg.step_stale;
+    const int pol = c->lin.amg_refresh;
+    const bool refresh = pol == ONSAS_AMG_REFRESH_ASSEMBLY ? c->amg.stale
+                         : pol == ONSAS_AMG_REFRESH_STEP   ? c->amg.step_stale
+                                                           : false;
     if (refresh || !c->amg.lam_set) {
       ONSAS_TRY(amg_numeric(c));
       c->amg.step_stale = false;
@@ -817,7 +832,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
                      c->lin.newton_tol, c->lin.eta_max > 0 ? c->lin.eta_max : 0.1,
                      c->lin.eta_first > 0 ? c->lin.eta_first : 1e-3);
   if (amg) {
-    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg));
+    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, false));
     hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
                        (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk, c->nn);
   }
@@ -829,12 +844,14 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
     return !(v && v[0] == '0');
   }();
   if (use_graph) ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
-  // First chunk: direct launches with every pcg_A bracketed by events on this
-  // stream (the sampled duration of the SpMV kernel); then graph replays.
-  const int nsample = std::min<int>(chunk_iters, (int)(sizeof(c->ev_spmv) / sizeof(hipEvent_t)) / 2) & ~1;
+  // First kSampleIters iterations: direct launches with every pcg_A bracketed by
+  // events on this stream (the sampled duration of the SpMV kernel); then graph
+  // replays of chunk_iters iterations.  kSampleIters is even, so the replays
+  // start at parity 0 as captured.
+  constexpr int kSampleIters = 2;
+  const int nsample = kSampleIters;
   for (int it = 0; it < nsample; ++it)
     launch_pcg_iteration<KT>(c, it & 1, c->ev_spmv[2 * it], c->ev_spmv[2 * it + 1]);
-  for (int it = nsample; it < chunk_iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
   ONSAS_HIP_TRY(c, hipGetLastError());
   bool first = true;
   for (;;) {

@@ -24,7 +24,7 @@ MAT_SVK, MAT_NEOHOOKEAN, MAT_LINEAR = 1, 2, 3
 STRAIN_ROTATED, STRAIN_GREEN = 0, 1
 LINSOLVE_AUTO, LINSOLVE_PCG, LINSOLVE_GMRES = 0, 1, 2
 PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1
-AMG_REFRESH_STEP, AMG_REFRESH_ASSEMBLY = 0, 1
+AMG_REFRESH_STEP, AMG_REFRESH_ASSEMBLY, AMG_REFRESH_ONCE = 0, 1, 2
 
 # every symbol declared in include/onsas_hip.h
 EXPORTED = (

@@ -19,7 +19,8 @@ from .materials import IsotropicLinearElastic, NeoHookean, SVK
 
 _SOLVERS = {"auto": _capi.LINSOLVE_AUTO, "pcg": _capi.LINSOLVE_PCG, "gmres": _capi.LINSOLVE_GMRES}
 _PRECONDS = {"block-jacobi": _capi.PRECOND_BLOCK_JACOBI, "amg": _capi.PRECOND_AMG}
-_AMG_REFRESH = {"step": _capi.AMG_REFRESH_STEP, "assembly": _capi.AMG_REFRESH_ASSEMBLY}
+_AMG_REFRESH = {"step": _capi.AMG_REFRESH_STEP, "assembly": _capi.AMG_REFRESH_ASSEMBLY,
+                "once": _capi.AMG_REFRESH_ONCE}
 
 
 @dataclass
@@ -30,8 +31,9 @@ class MI355XBackend:
     "auto" (PCG, GMRES if the tangent is indefinite), "pcg" or "gmres".
     preconditioner: "block-jacobi" (3x3 nodal blocks) or "amg" (smoothed-
     aggregation multigrid V-cycle with rigid-body modes and Chebyshev smoothing).
-    amg_refresh: "step" (coarse levels rebuilt once per load step) or "assembly"
-    (after every assembly); the finest level is always the current tangent."""
+    amg_refresh: "step" (coarse levels rebuilt once per load step), "assembly"
+    (after every assembly) or "once" (built from the first tangent only); the
+    finest level is always the current tangent."""
 
     precision: str = "mixed"
     rtol: float = None

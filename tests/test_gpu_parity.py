@@ -508,11 +508,12 @@ def test_amg_pcg_matches_direct(oracle, prec):
     assert its[_capi.PRECOND_AMG] < 0.2 * its[_capi.PRECOND_BLOCK_JACOBI]
 
 
-@pytest.mark.parametrize("refresh", ["step", "assembly"])
+@pytest.mark.parametrize("refresh", ["step", "assembly", "once"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_newton_with_amg(prec, refresh):
     """Compression cube with the AMG preconditioner, its coarse levels rebuilt
-    once per load step or after every assembly: the closed-form state."""
+    once per load step, after every assembly or only once: the closed-form
+    state."""
     print(f"[gpu] Newton with AMG {prec} refresh {refresh}", flush=True)
     s, sm, p = workloads.cube_compression(n=6, jitter=0.05, seed=2)
     nsteps = 4
