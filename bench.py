@@ -8,9 +8,16 @@ load steps to 15 % nominal strain, +-2 % interior-node jitter (seed 2).  A
 "step" is one full NonLinearStaticAnalysis solve (20 load steps) with
 NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-14, rel_res_force_tol=1e-8,
 max_iter=30)).  Inputs are resident in HBM before the timed region (the mesh,
-colouring and block pattern are built once, outside it).
+colouring, block pattern and multigrid pattern are built once, outside it).
+
+Headline: mixed precision, inexact-Newton forcing, PCG preconditioned by the
+smoothed-aggregation multigrid V-cycle whose coarse levels are built from the
+first tangent (refresh "once"; the finest level is always the current tangent).
+The other modes: all-FP64 with the same solver, block-Jacobi PCG, fixed inner
+tolerances.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-fp64] [--no-cpu]
+                    [--precond amg|block-jacobi] [--amg-refresh once|step|assembly]
 """
 from __future__ import annotations
 
@@ -33,14 +40,14 @@ def log(msg):
 
 
 def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False,
-             precond="block-jacobi"):
+             precond="block-jacobi", refresh="once"):
     from onsas_amd.analysis import (ConvergenceSettings, NewtonRaphson, NonLinearStaticAnalysis,
                                     ResidualsIterationStep, solve_)
     from onsas_amd.backend import GpuStaticState, MI355XBackend
     from onsas_amd import workloads
 
     be = MI355XBackend(precision=precision, maxiter=20000, store_stresses=False, forcing=forcing,
-                       preconditioner=precond)
+                       preconditioner=precond, amg_refresh=refresh)
     t0 = time.perf_counter()
     state = GpuStaticState(s, be)
     setup_s = time.perf_counter() - t0
@@ -139,8 +146,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-fixed", action="store_true", help="skip the fixed-rtol runs")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precond", default="block-jacobi", choices=["block-jacobi", "amg"],
-                    help="PCG preconditioner of every run")
+    ap.add_argument("--precond", default="amg", choices=["block-jacobi", "amg"],
+                    help="PCG preconditioner of the headline and FP64 runs")
+    ap.add_argument("--amg-refresh", default="once", choices=["once", "step", "assembly"])
+    ap.add_argument("--no-bj", action="store_true", help="skip the block-Jacobi comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this engine has no multi-GPU path); "
@@ -154,18 +163,21 @@ def main():
 
     runs = {}
     # headline: mixed precision with inexact-Newton forcing
+    pc = dict(precond=args.precond, refresh=args.amg_refresh)
     runs["mixed+forcing"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps,
-                                     forcing=True, precond=args.precond)
+                                     forcing=True, **pc)
     if not args.no_fp64:
         runs["fp64+forcing"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps,
-                                        forcing=True, precond=args.precond)
+                                        forcing=True, **pc)
+    if not args.no_bj and args.precond != "block-jacobi":
+        runs["mixed+forcing+block-jacobi"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup,
+                                                      args.nsteps, forcing=True,
+                                                      precond="block-jacobi")
     if not args.no_fixed:
         # the fixed inner tolerances: 1e-6 (mixed) and sqrt(eps) as in the reference (fp64)
-        runs["mixed"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps,
-                                 precond=args.precond)
+        runs["mixed"] = run_mode(s, sm, p, "mixed", args.steps, args.warmup, args.nsteps, **pc)
         if not args.no_fp64:
-            runs["fp64"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps,
-                                    precond=args.precond)
+            runs["fp64"] = run_mode(s, sm, p, "fp64", args.steps, args.warmup, args.nsteps, **pc)
 
     def asm_rates(r, kt):
         """Assembly throughput and its HBM roofline fraction from algorithmic
@@ -201,26 +213,38 @@ def main():
                     mode="gather" if tm["assembly_mode"] == 1 else "scatter")
 
     def spmv_roofline(r, kt):
+        """The fine-level tangent products: k_pcg_A (PCG: K blocks + col +
+        gathered z and p_old, written p_new and q) and, with the multigrid, the
+        cycle's four smoother products k_sell_cheb (K blocks + col, plus per dof
+        8, 3, 7 and 7 vector values read or written: v, b, block-Jacobi 3, r,
+        d1, x / v, r, r / x, b, block-Jacobi 3, r, d0 / d0, r, block-Jacobi 3,
+        x, z -- 6.25 on average)."""
         tm = r["timings"]
         nn, nnzb = r["nn"], r["nnzb"]
         n = 3 * nn
-        # pcg_A: K blocks + col + gathered z and p_old + written p_new and q
-        bytes_launch = nnzb * (9 * kt + 4) + 4 * n * kt
-        avg_ms = tm["spmv_ms"] / max(tm["spmv_launches"], 1)
-        ach = bytes_launch / (avg_ms * 1e-3) / 1e9
-        it = max(tm["pcg_iters"], 1)
-        return dict(bytes_per_launch=bytes_launch, avg_launch_ms=avg_ms, achieved_GBps=ach,
-                    pcg_ms_per_iter=tm["pcg_ms"] / it)
+        out = dict(pcg_ms_per_iter=tm["pcg_ms"] / max(tm["pcg_iters"], 1))
+        for name, ms, cnt, vec in (("k_pcg_A", tm["spmv_ms"], tm["spmv_launches"], 4),
+                                   ("k_sell_cheb", tm["cycle_spmv_ms"], tm["cycle_spmv_launches"],
+                                    6.25)):
+            if cnt <= 0:
+                continue
+            b = int(nnzb * (9 * kt + 4) + vec * n * kt)
+            avg = ms / cnt
+            out[name] = dict(bytes_per_launch=b, avg_launch_ms=avg,
+                             achieved_GBps=b / (avg * 1e-3) / 1e9, launches=cnt)
+        return out
 
     head = runs["mixed+forcing"]
     h_spmv = spmv_roofline(head, 4)
-    # HBM traffic per k_pcg_A launch from the committed rocprof summary of this
-    # workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-kernel calibration)
+    rk = "k_sell_cheb" if "k_sell_cheb" in h_spmv else "k_pcg_A"
+    hk = h_spmv[rk]
+    # HBM traffic per launch of that kernel from the committed rocprof summary of
+    # this workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-kernel calibration)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "newton_cube64_kernels.md")
     if args.n == 64 and os.path.exists(prof):
         import re
-        mt = re.search(r"traffic_bytes_per_launch k_pcg_A<float>: (\d+)", open(prof).read())
+        mt = re.search(r"traffic_bytes_per_launch %s<float>: (\d+)" % rk, open(prof).read())
         traffic = int(mt.group(1)) if mt else None
     result = {
         "metric": "Newton solve wall time & assembly Melem/s, mixed vs FP64 on 1 GPU at resid<=1e-8",
@@ -238,15 +262,19 @@ def main():
         "config": {"workload": f"NeoHookean compression cube {args.n}^3x6 tets ({ne} tets, "
                                f"{3 * sm.nn} dofs), {args.nsteps} load steps to 15% strain, "
                                f"mixed FP32 tangent/PCG + FP64 residual/update, inexact-Newton "
-                               f"forcing",
+                               f"forcing, PCG preconditioner {args.precond}"
+                               + (f" (coarse levels refresh {args.amg_refresh})"
+                                  if args.precond == "amg" else ""),
                    "tets": ne, "dofs": 3 * sm.nn, "load_steps": args.nsteps,
-                   "rel_res_force_tol": 1e-8},
-        "roofline": {"bound": "hbm", "achieved": round(h_spmv["achieved_GBps"], 1),
+                   "rel_res_force_tol": 1e-8, "preconditioner": args.precond},
+        "roofline": {"bound": "hbm", "achieved": round(hk["achieved_GBps"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(h_spmv["achieved_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_pcg_A (sliced-ELL 3x3 SpMV + p update, FP32)",
-                     "bytes_per_launch": h_spmv["bytes_per_launch"],
-                     "avg_launch_ms": round(h_spmv["avg_launch_ms"], 5)},
+                     "frac": round(hk["achieved_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": ("k_sell_cheb (fine-level sliced-ELL 3x3 tangent product + "
+                                "Chebyshev update, FP32)" if rk == "k_sell_cheb" else
+                                "k_pcg_A (sliced-ELL 3x3 SpMV + p update, FP32)"),
+                     "bytes_per_launch": hk["bytes_per_launch"],
+                     "avg_launch_ms": round(hk["avg_launch_ms"], 5)},
         "modes": {},
     }
     for name, r in runs.items():
@@ -261,6 +289,9 @@ def main():
     if "fp64" in runs and "mixed" in runs:
         result["speedup_mixed_vs_fp64_fixed_rtol"] = round(
             runs["fp64"]["wall_s"] / runs["mixed"]["wall_s"], 3)
+    if "mixed+forcing+block-jacobi" in runs:
+        result["speedup_vs_block_jacobi"] = round(
+            runs["mixed+forcing+block-jacobi"]["wall_s"] / head["wall_s"], 3)
     ref = runs.get("fp64") or runs.get("fp64+forcing")
     if ref is not None:
         d = np.abs(head["U_final"] - ref["U_final"]).max() / np.abs(ref["U_final"]).max()

@@ -115,10 +115,14 @@ typedef struct {
     int64_t n_steps;
     int64_t pcg_iters;
     int64_t gmres_iters;
-    double spmv_ms;          /* event-timed SpMV kernel launches (first chunk of each PCG) */
+    double spmv_ms;          /* event-timed SpMV kernel launches (second iteration of each
+                                PCG)                                                  */
     int64_t spmv_launches;   /* number of launches in spmv_ms                          */
     double elem_ms;          /* gather assembly: the per-element state pass alone     */
     int64_t assembly_mode;   /* 1 gather (default), 0 coloured scatter                */
+    double cycle_spmv_ms;    /* multigrid: event-timed fine-level smoother products of
+                                the V-cycle (all four, second iteration of each PCG) */
+    int64_t cycle_spmv_launches;
 } onsas_timings;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -172,7 +176,8 @@ int onsas_set_fixed_dofs(onsas_ctx* ctx, int64_t nfixed, const int64_t* dofs);
 int onsas_finalize(onsas_ctx* ctx);
 
 /* Replaces the linear_solve argument and default tolerances of solve!
- * (StructuralSolvers.jl:217-234). */
+ * (StructuralSolvers.jl:217-234).  With ONSAS_PRECOND_AMG on a finalized
+ * context it also builds the multigrid hierarchy's pattern (host work, once). */
 int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s);
 
 /* ---- per load step ------------------------------------------------------ */

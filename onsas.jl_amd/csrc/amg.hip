@@ -31,6 +31,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "engine.hpp"
 
 namespace onsas {
@@ -925,14 +927,21 @@ namespace {
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
-             const KT* x_base, KT* x_out, const PcgState* st) {
+             const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
+             hipEvent_t eb = nullptr) {
   hipStream_t s = c->stream;
   if (l == 0) {
     const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
-    hipLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s,
-                       c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
-                       (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2,
-                       d_out, x_base, x_out, c->nslices, c->nn, st);
+    if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
+      hipExtLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s, ea, eb,
+                            0, c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
+                            (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1,
+                            (KT)c2, d_out, x_base, x_out, c->nslices, c->nn, st);
+    else
+      hipLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s,
+                         c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
+                         (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2,
+                         d_out, x_base, x_out, c->nslices, c->nn, st);
   } else {
     const AmgLevel& L = c->amg.lev[l];
     if (L.nslices6 > 0)
@@ -1047,7 +1056,8 @@ void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, 
 }
 
 template <typename KT>
-int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready) {
+int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hipEvent_t* ev) {
+  auto evs = [&](int l, int j) { return l == 0 && ev ? ev[j] : nullptr; };
   Amg& A = c->amg;
   hipStream_t s = c->stream;
   const int nl = (int)A.lev.size();
@@ -1064,8 +1074,10 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready) {
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
     KT* x = (KT*)L.d_x;
     KT* r = (KT*)L.d_r;
-    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st);
-    op_cheb<KT>(c, l, (const KT*)L.d_d1, r, r, 0.0, 0.0, nullptr, nullptr, nullptr, st);
+    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
+                evs(l, 0), evs(l, 1));
+    op_cheb<KT>(c, l, (const KT*)L.d_d1, r, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 2),
+                evs(l, 3));
     AmgLevel& C = A.lev[l + 1];
     const bool coarsest = l + 2 == nl;
     const KT sc = coarsest ? KT(0) : (KT)cheb(C.lam).inv_theta;
@@ -1095,8 +1107,8 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready) {
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
                          L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
-    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st);
-    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st);
+    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 4), evs(l, 5));
+    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, evs(l, 6), evs(l, 7));
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
@@ -1108,10 +1120,11 @@ int amg_numeric(Ctx* c) {
   return c->precision == ONSAS_PRECISION_FP64 ? amg_numeric_t<double>(c) : amg_numeric_t<float>(c);
 }
 
-int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready) {
+int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
+                  hipEvent_t* ev) {
   return c->precision == ONSAS_PRECISION_FP64
-             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready)
-             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready);
+             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready, ev)
+             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev);
 }
 
 double amg_fine_smoother(Ctx* c, void** d0) {

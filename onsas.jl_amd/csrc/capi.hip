@@ -361,6 +361,10 @@ int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s) {
   c->lin = *s;
   if (c->lin.gmres_restart <= 0) c->lin.gmres_restart = 40;
   if (c->lin.check_every <= 0) c->lin.check_every = 16;
+  // the multigrid hierarchy's symbolic part (host work): here rather than in the
+  // first solve, when the model is complete; otherwise at the first solve
+  if (c->finalized && c->lin.precond == ONSAS_PRECOND_AMG && !c->amg.built)
+    ONSAS_TRY(onsas::amg_build(c));
   return ONSAS_OK;
 }
 

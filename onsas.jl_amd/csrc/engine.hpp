@@ -319,7 +319,10 @@ int amg_build(Ctx* c);                                          // hierarchy (ho
 int amg_numeric(Ctx* c);                                        // Galerkin, inverses, omega
 // x = M^-1 b; with d0_ready the first fine-level smoother step is already in
 // the buffer amg_fine_smoother names (fused into the caller's kernel)
-int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready);
+// ev (optional): 8 events, start/stop pairs stamped by the dispatches of the
+// four fine-level smoother products, in cycle order
+int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
+                  hipEvent_t* ev = nullptr);
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
 void amg_free(Ctx* c);
 

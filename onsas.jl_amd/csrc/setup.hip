@@ -465,7 +465,10 @@ int finalize_model(Ctx* c) {
 
   if (c->gather) ONSAS_TRY(launch_gather_init(c));
   for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
-  for (auto& e : c->ev_spmv) ONSAS_HIP_TRY(c, hipEventCreate(&e));
+  // timing-only events (read after a stream synchronize): no system-scope fence,
+  // so the stamped dispatch pays no cache writeback and invalidation
+  for (auto& e : c->ev_spmv)
+    ONSAS_HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->finalized = true;
   return ONSAS_OK;

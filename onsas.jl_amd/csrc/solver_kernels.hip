@@ -742,7 +742,7 @@ __global__ void k_residual_kt(const double* __restrict__ b, const KT* __restrict
 // One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
 // optional events around pcg_A.
 template <typename KT>
-void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
+void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEvent_t* evc = nullptr) {
   hipStream_t s = c->stream;
   KT* P[2] = {(KT*)c->d_pp0, (KT*)c->d_pp1};
   double* RZ[2] = {part(c, kPartRZ0), part(c, kPartRZ1)};
@@ -766,7 +766,7 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb) {
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
                        (const KT*)c->d_Dinv, (KT)s0, d0, c->np, c->chunk, c->nn);
-    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true);
+    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc);
     hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
                        (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
     return;
@@ -844,14 +844,21 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
     return !(v && v[0] == '0');
   }();
   if (use_graph) ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
-  // First kSampleIters iterations: direct launches with every pcg_A bracketed by
-  // events on this stream (the sampled duration of the SpMV kernel); then graph
-  // replays of chunk_iters iterations.  kSampleIters is even, so the replays
-  // start at parity 0 as captured.
+  // First kSampleIters iterations: direct launches, the last of them with its
+  // tangent products bracketed by dispatch-stamped events on this stream (the
+  // sampled durations; the launches right after the solve's host synchronizations
+  // start from an idle GPU and are not representative).  Then graph replays of
+  // chunk_iters iterations; kSampleIters is even, so they start at parity 0 as
+  // captured.
   constexpr int kSampleIters = 2;
   const int nsample = kSampleIters;
-  for (int it = 0; it < nsample; ++it)
-    launch_pcg_iteration<KT>(c, it & 1, c->ev_spmv[2 * it], c->ev_spmv[2 * it + 1]);
+  hipEvent_t* evc = c->ev_spmv + 32;  // the V-cycle's four fine-level products (AMG)
+  static_assert(32 + 8 <= sizeof(c->ev_spmv) / sizeof(hipEvent_t), "events");
+  for (int it = 0; it < nsample; ++it) {
+    const bool sampled = it == nsample - 1;
+    launch_pcg_iteration<KT>(c, it & 1, sampled ? c->ev_spmv[0] : nullptr,
+                             sampled ? c->ev_spmv[1] : nullptr, sampled ? evc : nullptr);
+  }
   ONSAS_HIP_TRY(c, hipGetLastError());
   bool first = true;
   for (;;) {
@@ -868,14 +875,19 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
     ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
     const PcgState& ps = c->h_sc->pcg;
     if (first) {
-      // iterations of the sampled chunk that did real work
+      // the sampled iteration, if it did real work
       const int worked = ps.breakdown ? ps.breakdown_k + 1 : (ps.done ? ps.iters : ps.k);
-      for (int it = 0; it < std::min(nsample, worked); ++it) {
+      if (worked >= nsample) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, c->ev_spmv[2 * it], c->ev_spmv[2 * it + 1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev_spmv[0], c->ev_spmv[1]) == hipSuccess) {
           c->tm.spmv_ms += ms;
           c->tm.spmv_launches++;
         }
+        for (int j = 0; amg && j < 4; ++j)
+          if (hipEventElapsedTime(&ms, evc[2 * j], evc[2 * j + 1]) == hipSuccess) {
+            c->tm.cycle_spmv_ms += ms;
+            c->tm.cycle_spmv_launches++;
+          }
       }
       first = false;
     }

@@ -63,7 +63,8 @@ class Timings(C.Structure):
                 ("pcg_ms", C.c_double), ("n_assemble", C.c_int64), ("n_steps", C.c_int64),
                 ("pcg_iters", C.c_int64), ("gmres_iters", C.c_int64),
                 ("spmv_ms", C.c_double), ("spmv_launches", C.c_int64),
-                ("elem_ms", C.c_double), ("assembly_mode", C.c_int64)]
+                ("elem_ms", C.c_double), ("assembly_mode", C.c_int64),
+                ("cycle_spmv_ms", C.c_double), ("cycle_spmv_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
