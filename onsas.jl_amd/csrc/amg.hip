@@ -13,8 +13,9 @@
 //   * smoothed prolongator P = (I - w D^-1 A) T, w = 4 / (3 lambda_max(D^-1 A))
 //     (block-CSR, bs x 6 blocks: row i has a block for every aggregate holding a
 //     neighbour of i); next level A_c = P^T (A P);
-//   * coarse levels have 6 dofs per node; coarsening goes on until at most 12
-//     nodes (72 dofs) remain, whose matrix is inverted densely in LDS.
+//   * coarse levels have 6 dofs per node; coarsening goes on until at most 300
+//     nodes (1800 dofs) remain, whose matrix is inverted densely (blocked
+//     Gauss-Jordan) and applied as one matrix-vector product.
 // Numerics (after every assembly, on the device): P, W = A P and P^T W, each
 // entry an FP64 sum over a fixed-order list built by the host (deterministic,
 // no atomics), 6x6 block-Jacobi inverses, the dense inverse of the coarsest
@@ -39,7 +40,7 @@ namespace onsas {
 
 namespace {
 
-constexpr int kMaxCoarseNodes = 12;  // coarsest level: <= 72 dofs, inverted in LDS
+constexpr int kMaxCoarseNodes = 300;  // coarsest level: <= 1800 dofs, inverted densely
 constexpr double kChebRatio = 30.0;  // smoothing interval [hi / 30, hi], hi = 1.1 lambda
 constexpr int kPowerFirst = 15;      // power-iteration steps, first estimate
 constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
@@ -235,17 +236,23 @@ __global__ void k_inv6(KT* __restrict__ A, const int32_t* __restrict__ diag, KT*
     for (int c = 0; c < 6; ++c) o[r * 6 + c] = (KT)M[r][6 + c];
 }
 
-// Coarsest level: block-CSR 6x6 -> dense FP64 (unit diagonal for unused dofs).
+// Coarsest level: block-CSR 6x6 -> dense FP64 (after k_zero_f64; unit diagonal
+// for unused dofs), one thread per (node row, component).
+__global__ void k_zero_f64(double* __restrict__ D, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    D[t] = 0;
+}
+
 template <typename KT>
 __global__ void k_dense_scatter(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                                 const KT* __restrict__ A, double* __restrict__ D, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (node row, comp)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nc = 6 * n;
   if (t >= nc) return;
   const int64_t I = t / 6;
   const int r = (int)(t % 6);
   double* drow = D + t * nc;
-  for (int64_t c = 0; c < nc; ++c) drow[c] = 0;
   for (int32_t k = row_ptr[I]; k < row_ptr[I + 1]; ++k) {
     const int64_t J = col[k];
 #pragma unroll
@@ -254,29 +261,120 @@ __global__ void k_dense_scatter(const int32_t* __restrict__ row_ptr, const int32
   if (drow[t] == 0) drow[t] = 1;
 }
 
-// In-place Gauss-Jordan inverse of an SPD dense matrix (n <= 72) in LDS, one
-// workgroup; the pivots are the diagonal (SPD: no pivoting needed).
-__global__ __launch_bounds__(256) void k_dense_inverse(double* __restrict__ D, int64_t n) {
-  __shared__ double M[72 * 72];
-  __shared__ double colk[72];
-  const int64_t nn = n * n;
-  for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) M[t] = D[t];
+// Blocked Gauss-Jordan inverse of the dense SPD coarsest matrix M (n x n, FP64,
+// in place; SPD: the pivots are the diagonal).  Per pivot block [k0, k0 + kb):
+// A11 <- A11^-1 (k_gj_diag), C <- A21 and A12 <- A11^-1 A12 (k_gj_rows),
+// then A22 <- A22 - C A12 and A21 <- -C A11^-1 (k_gj_update).
+constexpr int kGJ = 32;  // pivot block
+
+__global__ __launch_bounds__(256) void k_gj_diag(double* __restrict__ M, int64_t n, int64_t k0, int kb) {
+  __shared__ double S[kGJ][kGJ + 1];
+  __shared__ double colk[kGJ];
+  for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) S[t / kb][t % kb] = M[(k0 + t / kb) * n + k0 + t % kb];
   __syncthreads();
-  for (int64_t k = 0; k < n; ++k) {
-    for (int64_t r = threadIdx.x; r < n; r += blockDim.x) colk[r] = M[r * n + k];
+  for (int k = 0; k < kb; ++k) {
+    if ((int)threadIdx.x < kb) colk[threadIdx.x] = S[threadIdx.x][k];
     __syncthreads();
     const double piv = 1.0 / colk[k];
-    for (int64_t c = threadIdx.x; c < n; c += blockDim.x)
-      M[k * n + c] = (c == k) ? piv : M[k * n + c] * piv;
+    if ((int)threadIdx.x < kb) S[k][threadIdx.x] = ((int)threadIdx.x == k) ? piv : S[k][threadIdx.x] * piv;
     __syncthreads();
-    for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
-      const int64_t r = t / n, c = t - r * n;
+    for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) {
+      const int r = t / kb, c = t % kb;
       if (r == k) continue;
-      M[t] = (c == k) ? -colk[r] * piv : M[t] - colk[r] * M[k * n + c];
+      S[r][c] = (c == k) ? -colk[r] * piv : S[r][c] - colk[r] * S[k][c];
     }
     __syncthreads();
   }
-  for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) D[t] = M[t];
+  for (int t = threadIdx.x; t < kb * kb; t += blockDim.x) M[(k0 + t / kb) * n + k0 + t % kb] = S[t / kb][t % kb];
+}
+
+// Threads [0, n kGJ): C[i][b] = M[i][k0 + b] for rows i outside the block; then
+// one thread per column j outside the block: M[k0 + r][j] = (A11^-1 A12)[r][j].
+__global__ __launch_bounds__(256) void k_gj_rows(double* __restrict__ M, double* __restrict__ C, int64_t n,
+                                                 int64_t k0, int kb) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nA = n * kGJ;
+  if (t < nA) {
+    const int64_t i = t / kGJ;
+    const int b = (int)(t % kGJ);
+    if (b < kb && (i < k0 || i >= k0 + kb)) C[t] = M[i * n + k0 + b];
+    return;
+  }
+  const int64_t j = t - nA;
+  if (j >= n || (j >= k0 && j < k0 + kb)) return;
+  // every address below stays inside the block (b, r < kb; clamped otherwise)
+  double a[kGJ];
+#pragma unroll
+  for (int b = 0; b < kGJ; ++b) {
+    const int bb = b < kb ? b : 0;
+    a[b] = b < kb ? M[(k0 + bb) * n + j] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < kGJ; ++r) {
+    if (r >= kb) break;
+    const double* ai = M + (k0 + r) * n + k0;  // row r of A11^-1
+    double s = 0;
+#pragma unroll
+    for (int b = 0; b < kGJ; ++b) {
+      const int bb = b < kb ? b : 0;
+      s += (b < kb ? ai[bb] : 0.0) * a[b];
+    }
+    M[(k0 + r) * n + j] = s;
+  }
+}
+
+// Rows i outside the block: M[i][j] = (j in the block ? 0 : M[i][j]) -
+// sum_b C[i][b] M[k0 + b][j]; 64 x 64 output tiles, 4 x 4 per thread, C and the
+// block rows staged in LDS.
+__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ M, const double* __restrict__ C,
+                                                   int64_t n, int64_t k0, int kb) {
+  __shared__ double Cs[64][kGJ + 1];
+  __shared__ double Ms[kGJ][64 + 1];
+  const int64_t i0 = (int64_t)blockIdx.y * 64, j0 = (int64_t)blockIdx.x * 64;
+  for (int t = threadIdx.x; t < 64 * kGJ; t += 256) {  // clamped addresses, masked values
+    const int r = t / kGJ, b = t % kGJ;
+    const int64_t i = i0 + r;
+    const bool ok = i < n && b < kb && (i < k0 || i >= k0 + kb);
+    Cs[r][b] = ok ? C[(ok ? i : 0) * kGJ + b] : 0.0;
+  }
+  for (int t = threadIdx.x; t < kGJ * 64; t += 256) {
+    const int b = t / 64, cc = t % 64;
+    const int64_t j = j0 + cc;
+    const bool ok = b < kb && j < n;
+    Ms[b][cc] = ok ? M[(k0 + (ok ? b : 0)) * n + (ok ? j : 0)] : 0.0;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) acc[q][p] = 0;
+#pragma unroll 8
+  for (int b = 0; b < kGJ; ++b) {
+    double cv[4], mv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cv[q] = Cs[4 * ty + q][b];
+      mv[q] = Ms[b][4 * tx + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) acc[q][p] += cv[q] * mv[p];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = i0 + 4 * ty + q;
+    if (i >= n || (i >= k0 && i < k0 + kb)) continue;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int64_t j = j0 + 4 * tx + p;
+      if (j >= n) continue;
+      const bool inb = j >= k0 && j < k0 + kb;
+      M[i * n + j] = (inb ? 0.0 : M[i * n + j]) - acc[q][p];
+    }
+  }
 }
 
 template <typename KT>
@@ -540,17 +638,22 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
   xf[t] += acc;
 }
 
-// x = Ainv b on the coarsest level (dense), one thread per row.
+// x = Ainv b on the coarsest level (dense): one wave per row, lanes over its
+// columns, fixed reduction tree.
 template <typename KT>
-__global__ void k_dense_apply(const KT* __restrict__ Ainv, const KT* __restrict__ b,
-                              KT* __restrict__ x, int64_t nc, const PcgState* st) {
+__global__ __launch_bounds__(256) void k_dense_apply(const KT* __restrict__ Ainv,
+                                                     const KT* __restrict__ b, KT* __restrict__ x,
+                                                     int64_t nc, const PcgState* st) {
   if (pcg_stopped(st)) return;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int l = threadIdx.x & 63;
   if (r >= nc) return;
   const KT* a = Ainv + r * nc;
   KT s = 0;
-  for (int64_t c = 0; c < nc; ++c) s += a[c] * b[c];
-  x[r] = s;
+  for (int64_t c = l; c < nc; c += 64) s += a[c] * b[c];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (l == 0) x[r] = s;
 }
 
 // partial sums of v.v (power iteration), grid-stride, one partial per block
@@ -915,6 +1018,7 @@ int amg_build(Ctx* c) {
     ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));
   A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
+  ONSAS_TRY(alloc(c, (void**)&A.d_gjc, sizeof(double) * (size_t)A.nc * kGJ));
   ONSAS_TRY(alloc(c, &A.d_Ainv, c->kt * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, (void**)&A.d_part, sizeof(double) * 1024));
   A.built = true;
@@ -1031,9 +1135,18 @@ int amg_numeric_t(Ctx* c) {
     ONSAS_HIP_TRY(c, hipGetLastError());
   }
   const AmgLevel& Lc = A.lev[nl - 1];
-  hipLaunchKernelGGL(k_dense_scatter<KT>, dim3(nblocks(A.nc, 128)), dim3(128), 0, s, Lc.d_row_ptr,
+  const int64_t nc = A.nc;
+  hipLaunchKernelGGL(k_zero_f64, dim3(256), dim3(256), 0, s, A.d_dense, nc * nc);
+  hipLaunchKernelGGL(k_dense_scatter<KT>, dim3(nblocks(nc, 128)), dim3(128), 0, s, Lc.d_row_ptr,
                      Lc.d_col, (const KT*)Lc.d_A, A.d_dense, Lc.n);
-  hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(256), 0, s, A.d_dense, (int64_t)A.nc);
+  for (int64_t k0 = 0; k0 < nc; k0 += kGJ) {
+    const int kb = (int)std::min<int64_t>(kGJ, nc - k0);
+    hipLaunchKernelGGL(k_gj_diag, dim3(1), dim3(256), 0, s, A.d_dense, nc, k0, kb);
+    hipLaunchKernelGGL(k_gj_rows, dim3(nblocks(nc * kGJ + nc, 256)), dim3(256), 0, s, A.d_dense,
+                       A.d_gjc, nc, k0, kb);
+    hipLaunchKernelGGL(k_gj_update, dim3(nblocks(nc, 64), nblocks(nc, 64)), dim3(256), 0, s,
+                       A.d_dense, (const double*)A.d_gjc, nc, k0, kb);
+  }
   hipLaunchKernelGGL(k_to_kt<KT>, dim3(nblocks((int64_t)A.nc * A.nc, 256)), dim3(256), 0, s,
                      A.d_dense, (KT*)A.d_Ainv, (int64_t)A.nc * A.nc);
   ONSAS_HIP_TRY(c, hipGetLastError());
@@ -1089,7 +1202,7 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
   }
   // coarsest: exact
   const AmgLevel& Lc = A.lev[nl - 1];
-  hipLaunchKernelGGL(k_dense_apply<KT>, dim3(nblocks(A.nc, 64)), dim3(64), 0, s,
+  hipLaunchKernelGGL(k_dense_apply<KT>, dim3(nblocks(A.nc, 4)), dim3(256), 0, s,
                      (const KT*)A.d_Ainv, (const KT*)Lc.d_b, (KT*)Lc.d_x, (int64_t)A.nc, st);
   // up: prolongation, post-smoothing of the residual
   for (int l = nl - 2; l >= 0; --l) {
@@ -1143,7 +1256,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
                     (void*)L.d_s6_row, L.d_A6})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)A.d_dense, A.d_Ainv, (void*)A.d_part})
+  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part})
     if (p) (void)hipFree(p);
   A = Amg{};
 }

@@ -161,7 +161,8 @@ struct Amg {
   bool step_stale = true;  // ... to an earlier load step
   std::vector<AmgLevel> lev;
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
-  double* d_dense = nullptr;          // nc x nc FP64 work
+  double* d_dense = nullptr;          // nc x nc FP64 work (the inverse in place)
+  double* d_gjc = nullptr;            // nc x 32 FP64: a pivot block's column
   void* d_Ainv = nullptr;             // nc x nc KT
   double* d_part = nullptr;           // power-iteration partial sums
 };
