@@ -377,10 +377,13 @@ __global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ M, const
   }
 }
 
+// d (rows of ldc >= n, zero padded) = s (n x n)
 template <typename KT>
-__global__ void k_to_kt(const double* __restrict__ s, KT* __restrict__ d, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = (KT)s[i];
+__global__ void k_to_kt(const double* __restrict__ s, KT* __restrict__ d, int64_t n, int64_t ldc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * ldc) return;
+  const int64_t r = t / ldc, c = t % ldc;
+  d[t] = c < n ? (KT)s[r * n + c] : KT(0);
 }
 
 // ---- V-cycle kernels (all skip once the PCG has stopped) ---------------------------
@@ -643,14 +646,19 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
 template <typename KT>
 __global__ __launch_bounds__(256) void k_dense_apply(const KT* __restrict__ Ainv,
                                                      const KT* __restrict__ b, KT* __restrict__ x,
-                                                     int64_t nc, const PcgState* st) {
+                                                     int64_t nc, int64_t ldc, const PcgState* st) {
   if (pcg_stopped(st)) return;
   const int64_t r = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
   const int l = threadIdx.x & 63;
   if (r >= nc) return;
-  const KT* a = Ainv + r * nc;
+  // rows padded to ldc (a multiple of 4): 4 consecutive values per lane and load
+  const KT* a = Ainv + r * ldc;
   KT s = 0;
-  for (int64_t c = l; c < nc; c += 64) s += a[c] * b[c];
+  for (int64_t c = 4 * l; c < nc; c += 256) {
+    const KT* ap = (const KT*)__builtin_assume_aligned(a + c, 4 * sizeof(KT));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s += ap[j] * b[c + j < nc ? c + j : nc - 1];  // padding: ap[j] = 0
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (l == 0) x[r] = s;
@@ -1019,7 +1027,8 @@ int amg_build(Ctx* c) {
   A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, (void**)&A.d_gjc, sizeof(double) * (size_t)A.nc * kGJ));
-  ONSAS_TRY(alloc(c, &A.d_Ainv, c->kt * (size_t)A.nc * A.nc));
+  A.ldc = (A.nc + 3) & ~3;
+  ONSAS_TRY(alloc(c, &A.d_Ainv, c->kt * (size_t)A.nc * A.ldc));
   ONSAS_TRY(alloc(c, (void**)&A.d_part, sizeof(double) * 1024));
   A.built = true;
   A.stale = true;
@@ -1147,8 +1156,8 @@ int amg_numeric_t(Ctx* c) {
     hipLaunchKernelGGL(k_gj_update, dim3(nblocks(nc, 64), nblocks(nc, 64)), dim3(256), 0, s,
                        A.d_dense, (const double*)A.d_gjc, nc, k0, kb);
   }
-  hipLaunchKernelGGL(k_to_kt<KT>, dim3(nblocks((int64_t)A.nc * A.nc, 256)), dim3(256), 0, s,
-                     A.d_dense, (KT*)A.d_Ainv, (int64_t)A.nc * A.nc);
+  hipLaunchKernelGGL(k_to_kt<KT>, dim3(nblocks((int64_t)A.nc * A.ldc, 256)), dim3(256), 0, s,
+                     A.d_dense, (KT*)A.d_Ainv, (int64_t)A.nc, (int64_t)A.ldc);
   ONSAS_HIP_TRY(c, hipGetLastError());
   A.lam_set = true;
   A.stale = false;
@@ -1203,7 +1212,8 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
   // coarsest: exact
   const AmgLevel& Lc = A.lev[nl - 1];
   hipLaunchKernelGGL(k_dense_apply<KT>, dim3(nblocks(A.nc, 4)), dim3(256), 0, s,
-                     (const KT*)A.d_Ainv, (const KT*)Lc.d_b, (KT*)Lc.d_x, (int64_t)A.nc, st);
+                     (const KT*)A.d_Ainv, (const KT*)Lc.d_b, (KT*)Lc.d_x, (int64_t)A.nc,
+                     (int64_t)A.ldc, st);
   // up: prolongation, post-smoothing of the residual
   for (int l = nl - 2; l >= 0; --l) {
     AmgLevel& L = A.lev[l];

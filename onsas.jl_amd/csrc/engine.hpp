@@ -163,7 +163,8 @@ struct Amg {
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
   double* d_dense = nullptr;          // nc x nc FP64 work (the inverse in place)
   double* d_gjc = nullptr;            // nc x 32 FP64: a pivot block's column
-  void* d_Ainv = nullptr;             // nc x nc KT
+  int ldc = 0;                        // row length of d_Ainv (nc rounded up to 4)
+  void* d_Ainv = nullptr;             // nc x ldc KT, zero padded
   double* d_part = nullptr;           // power-iteration partial sums
 };
 

@@ -40,7 +40,7 @@ class MI355XBackend:
     maxiter: int = None
     solver: str = "auto"
     gmres_restart: int = 40
-    check_every: int = 16
+    check_every: int = None  # PCG iterations per replayed chunk (None: 16, multigrid 8)
     store_stresses: bool = True
     # inexact Newton: per-iteration inner tolerance (Eisenstat-Walker with a
     # floor at what the outer tolerance needs); rtol is then the tightest one
@@ -103,8 +103,9 @@ class GpuStaticState:
 
     def configure_linear_solver(self, newton_tol=0.0):
         b = self.backend
+        chunk = b.check_every or (8 if b.preconditioner == "amg" else 16)
         self.ctx.set_linear_solver(_SOLVERS[b.solver], b.rtol or 0.0, b.maxiter or 0,
-                                   b.gmres_restart, b.check_every, b.forcing, newton_tol,
+                                   b.gmres_restart, chunk, b.forcing, newton_tol,
                                    b.eta_max, b.eta_first, _PRECONDS[b.preconditioner],
                                    _AMG_REFRESH[b.amg_refresh])
 
