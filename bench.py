@@ -202,8 +202,14 @@ def main():
             # coloured scatter: conn 16 + dN/V 104 + u 96 + slots 64 + K block RMW + f_int RMW
             tot = ne * (16 + 104 + 96 + 64 + 2 * 16 * 9 * kt + 2 * 12 * 8)
             b_elem = b_gather = 0
+        # the reference algorithm's bytes per tet (coloured BSR scatter: conn 16,
+        # grad N and V 104, u 96, slot map 64, K block and f_int read-modify-write)
+        survey_b = 1624 if kt == 4 else 2776
         return dict(assemble_ms=asm_ms, elem_pass_ms=elem_ms, gather_pass_ms=gather_ms,
                     assembly_Melem_per_s=ne / (asm_ms * 1e-3) / 1e6,
+                    reference_algorithm_bytes_per_elem=survey_b,
+                    frac_of_peak_reference_algorithm_bytes=ne * survey_b / (asm_ms * 1e-3) / 1e9
+                    / HBM_PEAK_GBS,
                     bytes_per_assembly=tot, bytes_per_elem=tot / ne,
                     achieved_GBps=tot / (asm_ms * 1e-3) / 1e9,
                     frac_of_peak=tot / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -296,8 +302,15 @@ def main():
     if ref is not None:
         d = np.abs(head["U_final"] - ref["U_final"]).max() / np.abs(ref["U_final"]).max()
         result["mixed_vs_fp64_max_rel_disp_diff"] = float(d)
-    result["assembly_Melem_per_s"] = round(result["modes"]["mixed+forcing"]["assembly"]
-                                           ["assembly_Melem_per_s"], 1)
+    ha = result["modes"]["mixed+forcing"]["assembly"]
+    result["assembly_Melem_per_s"] = round(ha["assembly_Melem_per_s"], 1)
+    # assembly against HBM peak: with the reference algorithm's bytes per tet
+    # (coloured scatter, 1,624 B mixed) and with the bytes this gather design moves
+    result["assembly_roofline"] = {
+        "frac_reference_algorithm_bytes": round(ha["frac_of_peak_reference_algorithm_bytes"], 4),
+        "reference_algorithm_bytes_per_tet": ha["reference_algorithm_bytes_per_elem"],
+        "frac_gather_design_bytes": round(ha["frac_of_peak"], 4),
+        "gather_design_bytes_per_tet": round(ha["bytes_per_elem"], 1)}
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         result["cpu_baseline"] = cpu_baseline(nsteps=args.nsteps, n_full=args.n)
