@@ -26,15 +26,18 @@ pytestmark = pytest.mark.gpu
 TIGHT = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-16, rel_res_force_tol=1e-13, max_iter=30))
 
 
-def _backend(prec):
-    return MI355XBackend(precision=prec, maxiter=20000, store_stresses=False)
+def _backend(prec, pc="block-jacobi"):
+    return MI355XBackend(precision=prec, maxiter=20000, store_stresses=False, preconditioner=pc)
 
 
+@pytest.mark.parametrize("pc", ["block-jacobi", "amg"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("mat", ["nh", "svk"])
-def test_adjoint_matches_oracle(oracle, prec, mat):
+def test_adjoint_matches_oracle(oracle, prec, mat, pc):
+    """Forward Newton solve and refined adjoint solve with either PCG
+    preconditioner: dJ/dG against the oracle's direct adjoint."""
     import scipy.sparse.linalg as spla
-    print(f"[gpu] adjoint vs oracle {mat} {prec}", flush=True)
+    print(f"[gpu] adjoint vs oracle {mat} {prec} {pc}", flush=True)
     if mat == "nh":
         s, sm, p, G, _ = workloads.identification_cube(n=4, n_obs=50)
         code, p0 = oracle.MAT_NEOHOOKEAN, 8.3e5
@@ -48,7 +51,7 @@ def test_adjoint_matches_oracle(oracle, prec, mat):
         code, p0 = oracle.MAT_SVK, mat_obj.lam
         nsteps = 3
     sa = NonLinearStaticAnalysis(s, NSTEPS=nsteps)
-    sol, pullback = rrule(solve, sa, TIGHT, _backend(prec))
+    sol, pullback = rrule(solve, sa, TIGHT, _backend(prec, pc))
     U = sol.U[-1]
     Ubar = np.random.default_rng(5).normal(size=U.size)
     g = pullback(Ubar)["tets"]

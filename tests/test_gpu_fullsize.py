@@ -29,8 +29,9 @@ from onsas_amd.linear_analysis import LinearStaticAnalysis
 pytestmark = pytest.mark.gpu
 
 
-def test_truss_lattice_41_mixed_vs_fp64():
-    print("[gpu] truss lattice 41^3 full size", flush=True)
+@pytest.mark.parametrize("pc", ["block-jacobi", "amg"])
+def test_truss_lattice_41_mixed_vs_fp64(pc):
+    print(f"[gpu] truss lattice 41^3 full size, {pc}", flush=True)
     s1, sm = workloads.truss_lattice(n=40)
     # load scale from the linear (U = 0) response: mean top displacement 1 % of 4 m
     st = GpuStaticState(s1, MI355XBackend(precision="fp64", maxiter=50000))
@@ -48,7 +49,7 @@ def test_truss_lattice_41_mixed_vs_fp64():
         t0 = time.perf_counter()
         sols[prec] = solve(NonLinearStaticAnalysis(s, NSTEPS=20), alg,
                            MI355XBackend(precision=prec, maxiter=50000, forcing=True,
-                                         store_stresses=False))
+                                         store_stresses=False, preconditioner=pc))
         print(f"  {prec}: {time.perf_counter() - t0:.2f} s, newton {sum(sols[prec].iterations())}, "
               f"inner {sols[prec].linear_iterations()}", flush=True)
     for k in range(20):
@@ -69,12 +70,14 @@ def test_truss_lattice_41_mixed_vs_fp64():
     np.testing.assert_allclose(reaction, -load, rtol=0, atol=1e-6 * np.abs(load).max())
 
 
-def test_cylinder_full_size_lame():
-    print("[gpu] cylinder 24x192x96 full size", flush=True)
+@pytest.mark.parametrize("pc", ["block-jacobi", "amg"])
+def test_cylinder_full_size_lame(pc):
+    print(f"[gpu] cylinder 24x192x96 full size, {pc}", flush=True)
     s, sm = workloads.thick_cylinder()
     t0 = time.perf_counter()
     sol = solve(LinearStaticAnalysis(s, NSTEPS=2), None,
-                MI355XBackend(precision="mixed", maxiter=100000, store_stresses=False))
+                MI355XBackend(precision="mixed", maxiter=100000, store_stresses=False,
+                              preconditioner=pc))
     print(f"  solve {time.perf_counter() - t0:.2f} s, inner {sum(d['lin_iters'] for d in sol.newton_log)}",
           flush=True)
     U = sol.U[-1].reshape(-1, 3)
