@@ -120,8 +120,9 @@ typedef struct {
     int64_t spmv_launches;   /* number of launches in spmv_ms                          */
     double elem_ms;          /* gather assembly: the per-element state pass alone     */
     int64_t assembly_mode;   /* 1 gather (default), 0 coloured scatter                */
-    double cycle_spmv_ms;    /* multigrid: event-timed fine-level smoother products of
-                                the V-cycle (all four, second iteration of each PCG) */
+    double cycle_spmv_ms;    /* multigrid: event-timed exact fine-level products of the
+                                V-cycle (r = b - K x, both, second iteration of each
+                                PCG)                                                  */
     int64_t cycle_spmv_launches;
 } onsas_timings;
 

@@ -30,6 +30,7 @@
 // the PCG state says done, so replayed graphs cost nothing after convergence.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -61,6 +62,31 @@ __device__ inline T pick6(const T (&a)[6], int u) {
   r = u == 4 ? a[4] : r;
   r = u == 5 ? a[5] : r;
   return r;
+}
+
+// Value type of the smoother copy of the fine tangent: FP32 in FP64 mode (half
+// the bytes of the smoothing-only products).  Mixed mode has no copy: its
+// tangent is FP32 already, and a BF16 copy measured no faster overall (its
+// products are bound by load instructions, not bytes, and it costs a
+// conversion per assembly).
+template <typename KT>
+struct SmootherType {
+  using type = KT;
+};
+template <>
+struct SmootherType<double> {
+  using type = float;
+};
+
+template <typename KT, typename VT>
+__device__ inline KT ldv(VT x) {
+  return (KT)x;
+}
+
+__global__ void k_to_smoother_f64(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (float)s[i];
 }
 
 // Sum of a 6-vector over the 64 lanes of a wave (every lane gets the total).
@@ -393,11 +419,12 @@ __global__ void k_to_kt(const double* __restrict__ s, KT* __restrict__ d, int64_
 // (each output skipped when null, x_base null: zero).  b may be r_out and x_base
 // may be x_out: every row reads them before it writes them.
 
-// Fine level on the sliced-ELL tangent (one wave per slice).
-template <typename KT>
+// Fine level on the sliced-ELL tangent (one wave per slice); VT = KT for the
+// tangent itself, the smoother type for its lower-precision copy.
+template <typename KT, typename VT>
 __global__ __launch_bounds__(256) void k_sell_cheb(
     const int32_t* __restrict__ sell_ptr, const int32_t* __restrict__ slice_rounds, int64_t rstride,
-    const int32_t* __restrict__ scol, const KT* __restrict__ sval, const KT* __restrict__ v,
+    const int32_t* __restrict__ scol, const VT* __restrict__ sval, const KT* __restrict__ v,
     const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out,
     const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st) {
   if (pcg_stopped(st)) return;
@@ -408,15 +435,15 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
   const int32_t rounds = slice_rounds[s];
   const uint32_t rs = (uint32_t)rstride;
   const int32_t* cp = scol + b0 + lane;
-  const KT* vp = sval + 9u * (uint32_t)b0 + lane;
+  const VT* vp = sval + 9u * (uint32_t)b0 + lane;
   KT a0 = 0, a1 = 0, a2 = 0;
   for (int m = 0; m < rounds; ++m) {
     const uint32_t j = 3u * (uint32_t)cp[rs * m];
-    const KT* vv = vp + 9u * rs * m;
+    const VT* vv = vp + 9u * rs * m;
     const KT p0 = v[j], p1 = v[j + 1], p2 = v[j + 2];
-    a0 += vv[0] * p0 + vv[64] * p1 + vv[128] * p2;
-    a1 += vv[192] * p0 + vv[256] * p1 + vv[320] * p2;
-    a2 += vv[384] * p0 + vv[448] * p1 + vv[512] * p2;
+    a0 += ldv<KT>(vv[0]) * p0 + ldv<KT>(vv[64]) * p1 + ldv<KT>(vv[128]) * p2;
+    a1 += ldv<KT>(vv[192]) * p0 + ldv<KT>(vv[256]) * p1 + ldv<KT>(vv[320]) * p2;
+    a2 += ldv<KT>(vv[384]) * p0 + ldv<KT>(vv[448]) * p1 + ldv<KT>(vv[512]) * p2;
   }
 #pragma unroll
   for (int o = kSellRows; o < 64; o <<= 1) {
@@ -1024,6 +1051,8 @@ int amg_build(Ctx* c) {
   }
   for (void** v : {&A.lev[0].d_x, &A.lev[0].d_r, &A.lev[0].d_d0, &A.lev[0].d_d1, &A.lev[0].d_ev})
     ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));
+  if (f64) ONSAS_TRY(alloc(c, &A.d_Ks, sizeof(float) * 9 * (size_t)c->nsell));
+  A.ks_stale = true;
   A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, (void**)&A.d_gjc, sizeof(double) * (size_t)A.nc * kGJ));
@@ -1038,23 +1067,32 @@ int amg_build(Ctx* c) {
 namespace {
 
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
+// approx: at the fine level, use the smoother copy of the tangent (products
+// whose result only feeds a smoothing update)
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
-             hipEvent_t eb = nullptr) {
+             hipEvent_t eb = nullptr, bool approx = false) {
   hipStream_t s = c->stream;
   if (l == 0) {
+    using ST = typename SmootherType<KT>::type;
     const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
-    if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
-      hipExtLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s, ea, eb,
-                            0, c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
-                            (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1,
-                            (KT)c2, d_out, x_base, x_out, c->nslices, c->nn, st);
+    const dim3 grid(nblocks(c->nslices, 4));
+    if (approx && !std::is_same<ST, KT>::value)
+      hipLaunchKernelGGL((k_sell_cheb<KT, ST>), grid, dim3(256), 0, s, c->d_sell_ptr,
+                         c->d_slice_rounds, rstride, c->d_sell_col, (const ST*)c->amg.d_Ks, v, b,
+                         r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
+                         c->nslices, c->nn, st);
+    else if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
+      hipExtLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, ea, eb, 0, c->d_sell_ptr,
+                            c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
+                            r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
+                            c->nslices, c->nn, st);
     else
-      hipLaunchKernelGGL(k_sell_cheb<KT>, dim3(nblocks(c->nslices, 4)), dim3(256), 0, s,
-                         c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
-                         (const KT*)c->d_Ksell, v, b, r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2,
-                         d_out, x_base, x_out, c->nslices, c->nn, st);
+      hipLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, c->d_sell_ptr,
+                         c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
+                         r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
+                         c->nslices, c->nn, st);
   } else {
     const AmgLevel& L = c->amg.lev[l];
     if (L.nslices6 > 0)
@@ -1196,10 +1234,10 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
     KT* x = (KT*)L.d_x;
     KT* r = (KT*)L.d_r;
-    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
-                evs(l, 0), evs(l, 1));
-    op_cheb<KT>(c, l, (const KT*)L.d_d1, r, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 2),
-                evs(l, 3));
+    // x = d0 + d1 (smoother copy at the fine level), then the exact r = b - A x
+    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
+                nullptr, nullptr, true);
+    op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 0), evs(l, 1));
     AmgLevel& C = A.lev[l + 1];
     const bool coarsest = l + 2 == nl;
     const KT sc = coarsest ? KT(0) : (KT)cheb(C.lam).inv_theta;
@@ -1230,8 +1268,9 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
                          L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
-    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 4), evs(l, 5));
-    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, evs(l, 6), evs(l, 7));
+    // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
+    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
+    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, nullptr, nullptr, true);
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
@@ -1250,6 +1289,19 @@ int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_re
              : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev);
 }
 
+int amg_prepare(Ctx* c) {
+  Amg& A = c->amg;
+  if (!A.ks_stale) return ONSAS_OK;
+  const int64_t n = 9 * c->nsell;
+  if (c->precision == ONSAS_PRECISION_FP64) {
+    hipLaunchKernelGGL(k_to_smoother_f64, dim3(2048), dim3(256), 0, c->stream,
+                       (const double*)c->d_Ksell, (float*)A.d_Ks, n);
+    ONSAS_HIP_TRY(c, hipGetLastError());
+  }
+  A.ks_stale = false;
+  return ONSAS_OK;
+}
+
 double amg_fine_smoother(Ctx* c, void** d0) {
   *d0 = c->amg.lev[0].d_d0;
   return cheb(c->amg.lev[0].lam).inv_theta;
@@ -1266,7 +1318,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
                     (void*)L.d_s6_row, L.d_A6})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part})
+  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks})
     if (p) (void)hipFree(p);
   A = Amg{};
 }

@@ -173,7 +173,8 @@ int assemble_now(Ctx* c) {
   ONSAS_TRY(onsas::launch_assemble(c));
   if (!c->gather) ONSAS_TRY(onsas::launch_pack_sell(c));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  c->amg.stale = true;  // its Galerkin products belong to the previous tangent
+  c->amg.stale = true;     // its Galerkin products belong to the previous tangent
+  c->amg.ks_stale = true;  // ... and so does the smoother copy of K
   return ONSAS_OK;
 }
 

@@ -158,6 +158,8 @@ struct AmgLevel {
 struct Amg {
   bool built = false, lam_set = false;
   bool stale = true;       // the coarse levels belong to an earlier tangent
+  bool ks_stale = true;    // the smoother copy of the fine tangent is older than K
+  void* d_Ks = nullptr;    // that copy (FP64 mode): the sliced-ELL values in FP32
   bool step_stale = true;  // ... to an earlier load step
   std::vector<AmgLevel> lev;
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
@@ -321,11 +323,12 @@ int amg_build(Ctx* c);                                          // hierarchy (ho
 int amg_numeric(Ctx* c);                                        // Galerkin, inverses, omega
 // x = M^-1 b; with d0_ready the first fine-level smoother step is already in
 // the buffer amg_fine_smoother names (fused into the caller's kernel)
-// ev (optional): 8 events, start/stop pairs stamped by the dispatches of the
-// four fine-level smoother products, in cycle order
+// ev (optional): 4 events, start/stop pairs stamped by the dispatches of the
+// cycle's two exact fine-level products (r = b - A x), in cycle order
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
                   hipEvent_t* ev = nullptr);
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
+int amg_prepare(Ctx* c);  // refresh the smoother copy of the tangent (outside graphs)
 void amg_free(Ctx* c);
 
 // ---- solver_kernels.hip ----

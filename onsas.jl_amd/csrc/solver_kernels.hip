@@ -818,6 +818,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
       ONSAS_TRY(amg_numeric(c));
       c->amg.step_stale = false;
     }
+    ONSAS_TRY(amg_prepare(c));
     hipLaunchKernelGGL(k_pcg_init_r<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
                        (KT*)c->d_px, (KT*)c->d_pr, part(c, kPartRR), part(c, kPartBB), np, c->chunk,
                        c->nn);
@@ -852,7 +853,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
   // captured.
   constexpr int kSampleIters = 2;
   const int nsample = kSampleIters;
-  hipEvent_t* evc = c->ev_spmv + 32;  // the V-cycle's four fine-level products (AMG)
+  hipEvent_t* evc = c->ev_spmv + 32;  // the V-cycle's two exact fine-level products (AMG)
   static_assert(32 + 8 <= sizeof(c->ev_spmv) / sizeof(hipEvent_t), "events");
   for (int it = 0; it < nsample; ++it) {
     const bool sampled = it == nsample - 1;
@@ -883,7 +884,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
           c->tm.spmv_ms += ms;
           c->tm.spmv_launches++;
         }
-        for (int j = 0; amg && j < 4; ++j)
+        for (int j = 0; amg && j < 2; ++j)
           if (hipEventElapsedTime(&ms, evc[2 * j], evc[2 * j + 1]) == hipSuccess) {
             c->tm.cycle_spmv_ms += ms;
             c->tm.cycle_spmv_launches++;
