@@ -83,6 +83,22 @@ __device__ inline KT ldv(VT x) {
   return (KT)x;
 }
 
+// Value type of the smoothed prolongator P (and its P^T copy): FP16 in mixed
+// mode, FP32 in FP64 mode.  P's entries are O(1) (orthonormalised modes); W,
+// the coarse operators, restriction and prolongation all use the same rounded
+// values, so the coarse operator stays the exact Galerkin product of the P the
+// cycle applies.
+template <typename KT>
+struct ProlongType {
+  using type = _Float16;
+};
+template <>
+struct ProlongType<double> {
+  using type = float;
+};
+template <typename KT>
+using PT_t = typename ProlongType<KT>::type;
+
 __global__ void k_to_smoother_f64(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -121,8 +137,8 @@ __global__ __launch_bounds__(256) void k_sa_prolongator(
     const int32_t* __restrict__ p_row, const int32_t* __restrict__ p_col,
     const int32_t* __restrict__ agg, const int64_t* __restrict__ pk_ptr,
     const int32_t* __restrict__ pk, const int32_t* __restrict__ col, const KT* __restrict__ A,
-    const KT* __restrict__ Dinv, const KT* __restrict__ T, double w, KT* __restrict__ P,
-    const int32_t* __restrict__ rt_pos, KT* __restrict__ PT, int64_t nnz_p) {
+    const KT* __restrict__ Dinv, const KT* __restrict__ T, double w, PT_t<KT>* __restrict__ P,
+    const int32_t* __restrict__ rt_pos, PT_t<KT>* __restrict__ PT, int64_t nnz_p) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * nnz_p) return;
   const int64_t p = t / 6;
@@ -146,14 +162,14 @@ __global__ __launch_bounds__(256) void k_sa_prolongator(
   const KT* d = Dinv + (int64_t)BS * BS * i;
   const bool own = p_col[p] == agg[i];
   const KT* ti = T + (int64_t)BS * 6 * i + c;
-  KT* o = P + (int64_t)BS * 6 * p + c;
-  KT* ot = PT + (int64_t)BS * 6 * rt_pos[p] + c;
+  PT_t<KT>* o = P + (int64_t)BS * 6 * p + c;
+  PT_t<KT>* ot = PT + (int64_t)BS * 6 * rt_pos[p] + c;
 #pragma unroll
   for (int r = 0; r < BS; ++r) {
     double ds = 0;
 #pragma unroll
     for (int m = 0; m < BS; ++m) ds += (double)d[BS * r + m] * s[m];
-    const KT v = (KT)((own ? (double)ti[6 * r] : 0.0) - w * ds);
+    const PT_t<KT> v = (PT_t<KT>)(float)((own ? (double)ti[6 * r] : 0.0) - w * ds);
     o[6 * r] = v;
     ot[6 * r] = v;
   }
@@ -165,8 +181,9 @@ template <typename KT, int BS>
 __global__ __launch_bounds__(256) void k_sa_ap(const int64_t* __restrict__ wl_ptr,
                                                const int32_t* __restrict__ wl_k,
                                                const int32_t* __restrict__ wl_p,
-                                               const KT* __restrict__ A, const KT* __restrict__ P,
-                                               KT* __restrict__ W, int64_t nnz_w) {
+                                               const KT* __restrict__ A,
+                                               const PT_t<KT>* __restrict__ P, KT* __restrict__ W,
+                                               int64_t nnz_w) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * nnz_w) return;
   const int64_t w = t / 6;
@@ -176,7 +193,7 @@ __global__ __launch_bounds__(256) void k_sa_ap(const int64_t* __restrict__ wl_pt
   for (int r = 0; r < BS; ++r) s[r] = 0;
   for (int64_t e = wl_ptr[w]; e < wl_ptr[w + 1]; ++e) {
     const KT* a = A + (int64_t)BS * BS * wl_k[e];
-    const KT* pc = P + (int64_t)BS * 6 * wl_p[e] + c;
+    const PT_t<KT>* pc = P + (int64_t)BS * 6 * wl_p[e] + c;
     double pv[BS];
 #pragma unroll
     for (int q = 0; q < BS; ++q) pv[q] = (double)pc[6 * q];
@@ -196,7 +213,8 @@ template <typename KT, int BS>
 __global__ __launch_bounds__(256) void k_sa_ptap(const int64_t* __restrict__ gl_ptr,
                                                  const int32_t* __restrict__ gl_p,
                                                  const int32_t* __restrict__ gl_w,
-                                                 const KT* __restrict__ P, const KT* __restrict__ W,
+                                                 const PT_t<KT>* __restrict__ P,
+                                                 const KT* __restrict__ W,
                                                  KT* __restrict__ Ac, int64_t nnzb_c) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 6 * nnzb_c) return;
@@ -204,7 +222,7 @@ __global__ __launch_bounds__(256) void k_sa_ptap(const int64_t* __restrict__ gl_
   const int c = (int)(t % 6);
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t e = gl_ptr[g]; e < gl_ptr[g + 1]; ++e) {
-    const KT* pp = P + (int64_t)BS * 6 * gl_p[e];
+    const PT_t<KT>* pp = P + (int64_t)BS * 6 * gl_p[e];
     const KT* wc = W + (int64_t)BS * 6 * gl_w[e] + c;
     double wv[BS];
 #pragma unroll
@@ -607,9 +625,10 @@ __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
 template <typename KT, int BS, int WPA>
 __global__ __launch_bounds__(256) void k_sa_restrict(
     const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_row,
-    const KT* __restrict__ PT, const KT* __restrict__ rf, KT* __restrict__ bc,
+    const PT_t<KT>* __restrict__ PT, const KT* __restrict__ rf, KT* __restrict__ bc,
     const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc, const PcgState* st) {
   static_assert(WPA == 1 || WPA == 4, "one or four waves per aggregate");
+  using PV = PT_t<KT>;
   __shared__ KT sh[4][6];
   if (pcg_stopped(st)) return;
   const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / (64 * WPA);
@@ -618,7 +637,7 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
   KT acc[6] = {0, 0, 0, 0, 0, 0};
   for (int32_t e = rt_ptr[I] + l; e < rt_ptr[I + 1]; e += 64 * WPA) {
     const int64_t i = rt_row[e];
-    const KT* pp = (const KT*)__builtin_assume_aligned(PT + (int64_t)BS * 6 * e, 8);
+    const PV* pp = (const PV*)__builtin_assume_aligned(PT + (int64_t)BS * 6 * e, 2 * sizeof(PV));
     KT rv[BS];
 #pragma unroll
     for (int a = 0; a < BS; ++a) rv[a] = rf[BS * i + a];
@@ -651,8 +670,9 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
 // (node, dof).
 template <typename KT, int BS>
 __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
-                             const KT* __restrict__ P, const KT* __restrict__ xc,
+                             const PT_t<KT>* __restrict__ P, const KT* __restrict__ xc,
                              KT* __restrict__ xf, int64_t nf, const PcgState* st) {
+  using PV = PT_t<KT>;
   if (pcg_stopped(st)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= BS * nf) return;
@@ -660,7 +680,7 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
   const int a = (int)(t % BS);
   KT acc = 0;
   for (int32_t p = p_ptr[i]; p < p_ptr[i + 1]; ++p) {
-    const KT* pp = (const KT*)__builtin_assume_aligned(P + (int64_t)BS * 6 * p + 6 * a, 8);
+    const PV* pp = (const PV*)__builtin_assume_aligned(P + (int64_t)BS * 6 * p + 6 * a, 2 * sizeof(PV));
     const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)p_col[p], 8);
 #pragma unroll
     for (int u = 0; u < 6; ++u) acc += pp[u] * xj[u];
@@ -1020,8 +1040,9 @@ int amg_build(Ctx* c) {
     ONSAS_TRY(upload(c, &F.d_gl_ptr, gl_ptr));
     ONSAS_TRY(upload(c, &F.d_gl_p, gl_p));
     ONSAS_TRY(upload(c, &F.d_gl_w, gl_w));
-    ONSAS_TRY(alloc(c, &F.d_P, c->kt * bs * 6 * (size_t)nnz_p));
-    ONSAS_TRY(alloc(c, &F.d_PT, c->kt * bs * 6 * (size_t)nnz_p));
+    const size_t pbytes = f64 ? sizeof(float) : sizeof(_Float16);  // P's value type
+    ONSAS_TRY(alloc(c, &F.d_P, pbytes * bs * 6 * (size_t)nnz_p));
+    ONSAS_TRY(alloc(c, &F.d_PT, pbytes * bs * 6 * (size_t)nnz_p));
     ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)nnz_w));
     ONSAS_TRY(upload(c, &Lc.d_row_ptr, Lc.row_ptr));
     ONSAS_TRY(upload(c, &Lc.d_col, Lc.col));
@@ -1150,11 +1171,13 @@ void launch_products(Ctx* c, AmgLevel& F, AmgLevel& C, const KT* Af, const KT* D
   const double w = 4.0 / (3.0 * F.lam);
   hipLaunchKernelGGL((k_sa_prolongator<KT, BS>), dim3(nblocks(6 * F.nnz_p, 256)), dim3(256), 0, s,
                      F.d_p_row, F.d_p_col, F.d_agg, F.d_pk_ptr, F.d_pk, colf, Af, Df,
-                     (const KT*)F.d_T, w, (KT*)F.d_P, F.d_rt_pos, (KT*)F.d_PT, F.nnz_p);
+                     (const KT*)F.d_T, w, (PT_t<KT>*)F.d_P, F.d_rt_pos, (PT_t<KT>*)F.d_PT,
+                     F.nnz_p);
   hipLaunchKernelGGL((k_sa_ap<KT, BS>), dim3(nblocks(6 * F.nnz_w, 256)), dim3(256), 0, s,
-                     F.d_wl_ptr, F.d_wl_k, F.d_wl_p, Af, (const KT*)F.d_P, (KT*)F.d_W, F.nnz_w);
+                     F.d_wl_ptr, F.d_wl_k, F.d_wl_p, Af, (const PT_t<KT>*)F.d_P, (KT*)F.d_W,
+                     F.nnz_w);
   hipLaunchKernelGGL((k_sa_ptap<KT, BS>), dim3(nblocks(6 * C.nnzb, 256)), dim3(256), 0, s,
-                     F.d_gl_ptr, F.d_gl_p, F.d_gl_w, (const KT*)F.d_P, (const KT*)F.d_W,
+                     F.d_gl_ptr, F.d_gl_p, F.d_gl_w, (const PT_t<KT>*)F.d_P, (const KT*)F.d_W,
                      (KT*)C.d_A, C.nnzb);
 }
 
@@ -1207,11 +1230,11 @@ void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, 
                      const PcgState* st) {
   if (L.restrict_wide)
     hipLaunchKernelGGL((k_sa_restrict<KT, BS, 4>), dim3((unsigned)C.n), dim3(256), 0, c->stream,
-                       L.d_rt_ptr, L.d_rt_row, (const KT*)L.d_PT, r, (KT*)C.d_b,
+                       L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
   else
     hipLaunchKernelGGL((k_sa_restrict<KT, BS, 1>), dim3(nblocks(C.n, 4)), dim3(256), 0, c->stream,
-                       L.d_rt_ptr, L.d_rt_row, (const KT*)L.d_PT, r, (KT*)C.d_b,
+                       L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
 }
 
@@ -1264,10 +1287,10 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     KT* z = l == 0 ? z0 : x;
     if (l == 0)
       hipLaunchKernelGGL((k_sa_prolong<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
-                         L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
+                         L.d_p_ptr, L.d_p_col, (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
-                         L.d_p_ptr, L.d_p_col, (const KT*)L.d_P, (const KT*)C.d_x, x, L.n, st);
+                         L.d_p_ptr, L.d_p_col, (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
     op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
     op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, nullptr, nullptr, true);
