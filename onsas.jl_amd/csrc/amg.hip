@@ -83,6 +83,35 @@ __device__ inline KT ldv(VT x) {
   return (KT)x;
 }
 
+// Value type of the smoothing-only copy of the sliced-ELL coarse operators: BF16
+// (stored as uint16, round to nearest even; BF16 for the FP32 range, coarse
+// entries can exceed FP16's) in mixed mode, FP32 in FP64 mode.  These products
+// load 16 bytes per lane, so half the bytes is half the traffic.
+template <typename KT>
+struct CoarseSmootherType {
+  using type = uint16_t;
+};
+template <>
+struct CoarseSmootherType<double> {
+  using type = float;
+};
+
+template <>
+__device__ inline float ldv<float, uint16_t>(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+template <typename ST, typename KT>
+__device__ inline ST to_st(KT x) {
+  return (ST)x;
+}
+template <>
+__device__ inline uint16_t to_st<uint16_t, float>(float x) {
+  const uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
 // Value type of the smoothed prolongator P (and its P^T copy): FP16 in mixed
 // mode, FP32 in FP64 mode.  P's entries are O(1) (orthonormalised modes); W,
 // the coarse operators, restriction and prolongation all use the same rounded
@@ -534,10 +563,10 @@ __global__ __launch_bounds__(256) void k_bsr6_cheb(
 // lanes per row; every value load of a round reads 64 x 16 contiguous bytes.
 // After the reduction over the 8 lanes of a row, lanes ph < 6 finish component
 // ph of row s6_row[8 s + r] (rows sorted by length within windows; -1: none).
-template <typename KT>
+template <typename KT, typename VT>
 __global__ __launch_bounds__(256) void k_sell6_cheb(
     const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
-    const int32_t* __restrict__ s6_row, const KT* __restrict__ A6, const KT* __restrict__ v,
+    const int32_t* __restrict__ s6_row, const VT* __restrict__ A6, const KT* __restrict__ v,
     const KT* b, KT* r_out,
     const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
     KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
@@ -551,14 +580,14 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
     KT xv[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) xv[c] = xj[c];
-    const KT* a = A6 + 36 * 64 * (int64_t)q + 4 * l;
+    const VT* a = A6 + 36 * 64 * (int64_t)q + 4 * l;
 #pragma unroll
     for (int ch = 0; ch < 9; ++ch) {
-      const KT* ap = (const KT*)__builtin_assume_aligned(a + 256 * ch, 16);
+      const VT* ap = (const VT*)__builtin_assume_aligned(a + 256 * ch, 4 * sizeof(VT));
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = 4 * ch + j;
-        acc[e / 6] += ap[j] * xv[e % 6];
+        acc[e / 6] += ldv<KT>(ap[j]) * xv[e % 6];
       }
     }
   }
@@ -588,7 +617,9 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
 // Block-CSR -> sliced-ELL values: one thread per (round slot, value).
 template <typename KT>
 __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restrict__ A,
-                           KT* __restrict__ A6, int64_t nslots) {
+                           KT* __restrict__ A6, typename CoarseSmootherType<KT>::type* __restrict__ A6s,
+                           int64_t nslots) {
+  using ST = typename CoarseSmootherType<KT>::type;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 36 * nslots) return;
   const int64_t slot = t / 36;  // 64 q + l
@@ -596,7 +627,10 @@ __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restr
   const int64_t q = slot >> 6;
   const int l = (int)(slot & 63);
   const int32_t k = s6_src[slot];
-  A6[((9 * q + e / 4) * 64 + l) * 4 + (e & 3)] = k >= 0 ? A[36 * (int64_t)k + e] : KT(0);
+  const KT v = k >= 0 ? A[36 * (int64_t)k + e] : KT(0);
+  const int64_t at = ((9 * q + e / 4) * 64 + l) * 4 + (e & 3);
+  A6[at] = v;
+  A6s[at] = to_st<ST>(v);
 }
 
 // y = s D^-1 v, one thread per (node, component).
@@ -1054,6 +1088,8 @@ int amg_build(Ctx* c) {
       ONSAS_TRY(upload(c, &Lc.d_s6_src, s6_src));
       ONSAS_TRY(upload(c, &Lc.d_s6_row, s6_row));
       ONSAS_TRY(alloc(c, &Lc.d_A6, c->kt * 36 * 64 * (size_t)Lc.nrounds6));
+      ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(float) : sizeof(uint16_t)) * 36 * 64 *
+                                        (size_t)Lc.nrounds6));
     }
     ONSAS_TRY(alloc(c, &Lc.d_Dinv, c->kt * 36 * (size_t)na));
     for (void** v : {&Lc.d_b, &Lc.d_x, &Lc.d_r, &Lc.d_d0, &Lc.d_d1, &Lc.d_ev})
@@ -1088,8 +1124,9 @@ int amg_build(Ctx* c) {
 namespace {
 
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
-// approx: at the fine level, use the smoother copy of the tangent (products
-// whose result only feeds a smoothing update)
+// approx: the product's result only feeds a smoothing update; it may use the
+// smoother copy of the operator (fine level in FP64 mode, sliced-ELL coarse
+// levels)
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
@@ -1116,8 +1153,14 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
                          c->nslices, c->nn, st);
   } else {
     const AmgLevel& L = c->amg.lev[l];
-    if (L.nslices6 > 0)
-      hipLaunchKernelGGL(k_sell6_cheb<KT>, dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
+    using CT = typename CoarseSmootherType<KT>::type;
+    if (L.nslices6 > 0 && approx)
+      hipLaunchKernelGGL((k_sell6_cheb<KT, CT>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
+                         L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const CT*)L.d_A6s, v, b, r_out,
+                         (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6,
+                         L.n, st);
+    else if (L.nslices6 > 0)
+      hipLaunchKernelGGL((k_sell6_cheb<KT, KT>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
                          L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const KT*)L.d_A6, v, b, r_out,
                          (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6,
                          L.n, st);
@@ -1200,7 +1243,8 @@ int amg_numeric_t(Ctx* c) {
                          (KT*)C.d_Dinv, C.n);
       if (C.nslices6 > 0)
         hipLaunchKernelGGL(k_to_sell6<KT>, dim3(nblocks(36 * 64 * C.nrounds6, 256)), dim3(256), 0,
-                           s, C.d_s6_src, (const KT*)C.d_A, (KT*)C.d_A6, 64 * C.nrounds6);
+                           s, C.d_s6_src, (const KT*)C.d_A, (KT*)C.d_A6,
+                           (typename CoarseSmootherType<KT>::type*)C.d_A6s, 64 * C.nrounds6);
     }
     ONSAS_HIP_TRY(c, hipGetLastError());
   }
@@ -1339,7 +1383,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_rt_ptr, (void*)L.d_rt_row, (void*)L.d_rt_pos, L.d_PT, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
                     (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w,
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
-                    (void*)L.d_s6_row, L.d_A6})
+                    (void*)L.d_s6_row, L.d_A6, L.d_A6s})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks})
     if (p) (void)hipFree(p);

@@ -130,6 +130,7 @@ struct AmgLevel {
   int64_t nslices6 = 0, nrounds6 = 0;
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
   void* d_A6 = nullptr;
+  void* d_A6s = nullptr;              // its smoothing-only copy (BF16 mixed, FP32 FP64)
   double lam = 0;                     // lambda_max(D^-1 A), power iteration
   void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)
   // cycle vectors (KT, bs n): right-hand side, iterate, residual, Chebyshev steps
