@@ -899,7 +899,9 @@ void group_row(std::vector<Trip>& trips, std::vector<int32_t>& cols, std::vector
 // Build the hierarchy's symbolic part (host) and allocate its device arrays.
 int amg_build(Ctx* c) {
   Amg& A = c->amg;
+  const int64_t gen = A.gen;  // monotonic across rebuilds (graph cache key)
   A = Amg{};
+  A.gen = gen + 1;
   const int64_t nn = c->nn;
   const bool f64 = c->precision == ONSAS_PRECISION_FP64;
   AmgLevel L0;
@@ -1266,6 +1268,7 @@ int amg_numeric_t(Ctx* c) {
   ONSAS_HIP_TRY(c, hipGetLastError());
   A.lam_set = true;
   A.stale = false;
+  ++A.gen;  // new smoothing coefficients: graphs captured before are stale
   return ONSAS_OK;
 }
 

@@ -224,7 +224,8 @@ void onsas_destroy(onsas_ctx* ctx) {
   if (!ctx) return;
   Ctx* c = &ctx->impl;
   (void)hipStreamSynchronize(c->stream);
-  if (c->pcg_graph) (void)hipGraphExecDestroy(c->pcg_graph);
+  for (auto& g : c->pcg_graph)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   onsas::amg_free(c);
   for (auto& g : c->groups)
     for (void* p : {(void*)g.d_conn, (void*)g.d_geo, (void*)g.d_slot, (void*)g.d_G, (void*)g.d_perm,

@@ -158,6 +158,8 @@ struct AmgLevel {
 
 struct Amg {
   bool built = false, lam_set = false;
+  int64_t gen = 0;         // numerics generation: captured launches bake in its
+                           // smoothing coefficients
   bool stale = true;       // the coarse levels belong to an earlier tangent
   bool ks_stale = true;    // the smoother copy of the fine tangent is older than K
   void* d_Ks = nullptr;    // that copy (FP64 mode): the sliced-ELL values in FP32
@@ -275,10 +277,15 @@ struct Ctx {
   // linear solver settings
   onsas_linear_settings lin{};
 
-  // PCG graph (a chunk of iterations; keyed by its length and preconditioner)
-  hipGraphExec_t pcg_graph = nullptr;
-  int pcg_graph_iters = 0;
-  int pcg_graph_precond = -1;
+  // PCG graphs: [0] the opening iterations of a solve, [1] the repeated chunk;
+  // each keyed by its length, preconditioner and multigrid generation
+  struct PcgGraph {
+    hipGraphExec_t exec = nullptr;
+    int iters = 0;
+    int precond = -1;
+    int64_t gen = -1;
+  } pcg_graph[2];
+  int64_t pcg_solves = 0;  // PCG solves so far (every kSampleEvery-th is timed)
 
   // aggregation multigrid preconditioner (built on its first use)
   Amg amg;

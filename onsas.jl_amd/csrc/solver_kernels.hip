@@ -776,13 +776,17 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
                      part(c, kPartPQ), RZ[1 - par], part(c, kPartRR), st, c->np, c->chunk, c->nn);
 }
 
+// Graph slot `slot` holding `iters` PCG iterations (even: parity 0 to 0), captured
+// again when the length, preconditioner or multigrid generation changed (the
+// launches bake in the smoothing coefficients).
 template <typename KT>
-int build_pcg_graph(Ctx* c, int iters) {
-  if (c->pcg_graph && c->pcg_graph_iters == iters && c->pcg_graph_precond == c->lin.precond)
-    return ONSAS_OK;
-  if (c->pcg_graph) {
-    hipGraphExecDestroy(c->pcg_graph);
-    c->pcg_graph = nullptr;
+int build_pcg_graph(Ctx* c, int slot, int iters) {
+  const int64_t gen = c->lin.precond == ONSAS_PRECOND_AMG ? c->amg.gen : 0;
+  Ctx::PcgGraph& g = c->pcg_graph[slot];
+  if (g.exec && g.iters == iters && g.precond == c->lin.precond && g.gen == gen) return ONSAS_OK;
+  if (g.exec) {
+    hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
   }
   hipStream_t s = c->stream;
   ONSAS_HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -790,11 +794,12 @@ int build_pcg_graph(Ctx* c, int iters) {
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (e != hipSuccess) return set_err(c, ONSAS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&c->pcg_graph, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   if (e != hipSuccess) return set_err(c, ONSAS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-  c->pcg_graph_iters = iters;
-  c->pcg_graph_precond = c->lin.precond;
+  g.iters = iters;
+  g.precond = c->lin.precond;
+  g.gen = gen;
   return ONSAS_OK;
 }
 
@@ -844,28 +849,39 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
     const char* v = std::getenv("ONSAS_PCG_GRAPH");
     return !(v && v[0] == '0');
   }();
-  if (use_graph) ONSAS_TRY(build_pcg_graph<KT>(c, chunk_iters));
-  // First kSampleIters iterations: direct launches, the last of them with its
-  // tangent products bracketed by dispatch-stamped events on this stream (the
-  // sampled durations; the launches right after the solve's host synchronizations
-  // start from an idle GPU and are not representative).  Then graph replays of
-  // chunk_iters iterations; kSampleIters is even, so they start at parity 0 as
-  // captured.
-  constexpr int kSampleIters = 2;
-  const int nsample = kSampleIters;
+  // The opening kFirstIters iterations, then a convergence check, then chunks
+  // of chunk_iters iterations with a check after each (both even, so every
+  // replay starts at parity 0 as captured).  Every kSampleEvery-th solve (every
+  // solve without graphs) launches the opening iterations directly, the last of
+  // them with its tangent products bracketed by dispatch-stamped events on this
+  // stream: the sampled durations (the launches right after the solve's host
+  // synchronizations start from an idle GPU and are not representative).  The
+  // other solves replay the opening iterations as a graph too.
+  constexpr int kFirstIters = 2;
+  constexpr int kSampleEvery = 8;
+  const bool sample = !use_graph || (c->pcg_solves++ % kSampleEvery) == 0;
+  if (use_graph) {
+    ONSAS_TRY(build_pcg_graph<KT>(c, 0, kFirstIters));
+    ONSAS_TRY(build_pcg_graph<KT>(c, 1, chunk_iters));
+  }
+  const int nsample = kFirstIters;
   hipEvent_t* evc = c->ev_spmv + 32;  // the V-cycle's two exact fine-level products (AMG)
   static_assert(32 + 8 <= sizeof(c->ev_spmv) / sizeof(hipEvent_t), "events");
-  for (int it = 0; it < nsample; ++it) {
-    const bool sampled = it == nsample - 1;
-    launch_pcg_iteration<KT>(c, it & 1, sampled ? c->ev_spmv[0] : nullptr,
-                             sampled ? c->ev_spmv[1] : nullptr, sampled ? evc : nullptr);
+  if (sample) {
+    for (int it = 0; it < nsample; ++it) {
+      const bool sampled = it == nsample - 1;
+      launch_pcg_iteration<KT>(c, it & 1, sampled ? c->ev_spmv[0] : nullptr,
+                               sampled ? c->ev_spmv[1] : nullptr, sampled ? evc : nullptr);
+    }
+  } else {
+    ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph[0].exec, s));
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   bool first = true;
   for (;;) {
     if (!first) {
       if (use_graph) {
-        ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph, s));
+        ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph[1].exec, s));
       } else {
         for (int it = 0; it < chunk_iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
         ONSAS_HIP_TRY(c, hipGetLastError());
@@ -875,7 +891,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
                                     hipMemcpyDeviceToHost, s));
     ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
     const PcgState& ps = c->h_sc->pcg;
-    if (first) {
+    if (first && sample) {
       // the sampled iteration, if it did real work
       const int worked = ps.breakdown ? ps.breakdown_k + 1 : (ps.done ? ps.iters : ps.k);
       if (worked >= nsample) {
@@ -890,8 +906,8 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, int* iters, double* r
             c->tm.cycle_spmv_launches++;
           }
       }
-      first = false;
     }
+    first = false;
     if (ps.done || ps.breakdown) break;
   }
   const PcgState& ps = c->h_sc->pcg;
