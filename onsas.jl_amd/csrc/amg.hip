@@ -178,6 +178,7 @@ __global__ __launch_bounds__(256) void k_sa_prolongator(
   for (int r = 0; r < BS; ++r) s[r] = 0;
   for (int64_t e = pk_ptr[p]; e < pk_ptr[p + 1]; ++e) {
     const int64_t k = pk[e];
+    if (k < 0) continue;  // own-aggregate block of a row without one in the pattern
     const KT* a = A + (int64_t)BS * BS * k;
     const KT* tj = T + (int64_t)BS * 6 * col[k] + c;
     double tc[BS];
@@ -251,6 +252,7 @@ __global__ __launch_bounds__(256) void k_sa_ptap(const int64_t* __restrict__ gl_
   const int c = (int)(t % 6);
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t e = gl_ptr[g]; e < gl_ptr[g + 1]; ++e) {
+    if (gl_p[e] < 0) continue;  // diagonal block of a row without a product in it
     const PT_t<KT>* pp = P + (int64_t)BS * 6 * gl_p[e];
     const KT* wc = W + (int64_t)BS * 6 * gl_w[e] + c;
     double wv[BS];
@@ -969,7 +971,14 @@ int amg_build(Ctx* c) {
     std::vector<int64_t> pk_ptr(1, 0);
     for (int64_t i = 0; i < n; ++i) {
       trips.clear();
-      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) trips.push_back({agg[cl[k]], k, 0});
+      bool own = false;
+      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        trips.push_back({agg[cl[k]], k, 0});
+        own |= agg[cl[k]] == agg[i];
+      }
+      // every row gets a block in its own aggregate (a node without a diagonal
+      // block in the pattern -- an isolated, fixed node -- through an empty entry)
+      if (!own) trips.push_back({agg[i], -1, 0});
       const size_t before = p_col.size();
       group_row(trips, p_col, pk_ptr, pk, nullptr);
       p_row.insert(p_row.end(), p_col.size() - before, (int32_t)i);
@@ -1010,10 +1019,15 @@ int amg_build(Ctx* c) {
     std::vector<int64_t> gl_ptr(1, 0);
     for (int64_t I = 0; I < na; ++I) {
       trips.clear();
+      bool has_diag = false;
       for (int32_t e = rt_ptr[I]; e < rt_ptr[I + 1]; ++e) {
         const int32_t p = rt[e], i = p_row[p];
-        for (int32_t w = w_ptr[i]; w < w_ptr[i + 1]; ++w) trips.push_back({w_col[w], p, w});
+        for (int32_t w = w_ptr[i]; w < w_ptr[i + 1]; ++w) {
+          trips.push_back({w_col[w], p, w});
+          has_diag |= w_col[w] == I;
+        }
       }
+      if (!has_diag) trips.push_back({(int32_t)I, -1, -1});  // zero block: unit diagonal later
       const size_t before = Lc.col.size();
       group_row(trips, Lc.col, gl_ptr, gl_p, &gl_w);
       for (size_t k = before; k < Lc.col.size(); ++k)

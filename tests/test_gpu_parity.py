@@ -543,6 +543,57 @@ def test_amg_large_hierarchy(prec):
     c.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_amg_edge_cases(prec):
+    """Multigrid on models the cube tests do not reach: tets + trusses with a
+    fully fixed node that no element touches (no block of its own in the
+    pattern), and a one-cell cube (8 nodes: the first coarse level is already
+    the dense one).  The solves meet the same residual bounds as block-Jacobi."""
+    print(f"[gpu] AMG edge cases {prec}", flush=True)
+    rng = np.random.default_rng(8)
+    sm = meshes.cube(3, jitter=0.1, seed=4)
+    X = np.vstack([sm.coords, [[2.0, 2.0, 2.0]]])  # the isolated node
+    nn = len(X)
+    bars = np.array([[0, 5], [3, 17], [10, 40], [21, 63], [1, 2]], dtype=np.int32)
+    A = np.array([1e-3, 2e-3, 1.5e-3, 1e-3, 3e-3])
+    fixed = np.concatenate([np.arange(3), [3 * 4 + 1], np.arange(3 * (nn - 1), 3 * nn)])
+    c = _capi.Context(PREC[prec])
+    c.set_nodes(X)
+    c.add_tet_group(sm.tets, _capi.MAT_SVK, MATS["svk"][1], None)
+    c.add_truss_group(bars, 1, 2e7, A)
+    c.set_fixed_dofs(fixed)
+    c.finalize()
+    c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=1e-10 if prec == "fp64" else 1e-6, maxiter=500,
+                        precond=_capi.PRECOND_AMG)
+    U = rng.normal(scale=0.02, size=3 * nn)
+    U[fixed] = 0
+    c.set_U(U)
+    c.assemble()
+    b = rng.normal(size=3 * nn)
+    x, it, rr = c.linear_solve(b)
+    free = np.setdiff1d(np.arange(3 * nn), fixed)
+    Kff = c.tangent_bsr().tocsr()[free][:, free]
+    res = np.linalg.norm(Kff @ x[free] - b[free]) / np.linalg.norm(b[free])
+    print(f"  tets + trusses + isolated node: {it} iterations, residual {res:.1e}", flush=True)
+    assert res <= (1e-7 if prec == "fp64" else 5e-4)
+    assert np.all(x[fixed] == 0)
+    c.close()
+    s, sm1 = workloads.cube_extension(n=1)
+    c = _ctx(prec, sm1.coords, sm1.tets, "svk", fixed=s.fixed_dofs)
+    c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=1e-10 if prec == "fp64" else 1e-6, maxiter=100,
+                        precond=_capi.PRECOND_AMG)
+    c.set_U(np.zeros(3 * sm1.nn))
+    c.assemble()
+    b = rng.normal(size=3 * sm1.nn)
+    x, it, rr = c.linear_solve(b)
+    free = np.setdiff1d(np.arange(3 * sm1.nn), s.fixed_dofs)
+    Kff = c.tangent_bsr().tocsr()[free][:, free]
+    res = np.linalg.norm(Kff @ x[free] - b[free]) / np.linalg.norm(b[free])
+    print(f"  one-cell cube: {it} iterations, residual {res:.1e}", flush=True)
+    assert res <= (1e-9 if prec == "fp64" else 1e-5) and it <= 20  # two levels, 6 coarse modes
+    c.close()
+
+
 @pytest.mark.parametrize("refresh", ["step", "assembly", "once"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_newton_with_amg(prec, refresh):
