@@ -249,7 +249,8 @@ def main():
     prof = os.path.join(ROOT, "profiles", "newton_cube64_kernels.md")
     if args.n == 64 and os.path.exists(prof):
         import re
-        mt = re.search(r"traffic_bytes_per_launch %s<float>: (\d+)" % rk, open(prof).read())
+        full = {"k_sell_cheb": "k_sell_cheb<float, float>", "k_pcg_A": "k_pcg_A<float>"}[rk]
+        mt = re.search(r"traffic_bytes_per_launch %s: (\d+)" % re.escape(full), open(prof).read())
         traffic = int(mt.group(1)) if mt else None
     result = {
         "metric": "Newton solve wall time & assembly Melem/s, mixed vs FP64 on 1 GPU at resid<=1e-8",
