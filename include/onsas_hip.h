@@ -218,6 +218,25 @@ int onsas_get_pattern(onsas_ctx* ctx, int32_t* row_ptr, int32_t* col);
  * fixed dofs hold the identity (the free-free block equals K[free, free] of
  * NonLinearStaticAnalyses.jl:120). */
 int onsas_get_tangent(onsas_ctx* ctx, double* vals);
+
+/* Multigrid hierarchy inspection.  No counterpart in the reference (its CG is
+ * unpreconditioned); for tests and diagnostics.  Available once a PCG solve
+ * with ONSAS_PRECOND_AMG has built the hierarchy (0 levels before).  Level 0
+ * is the tangent (onsas_get_tangent); level l >= 1 has 6 dofs per node. */
+int onsas_amg_levels(onsas_ctx* ctx, int32_t* nlevels);
+/* Level l: nodes, dofs per node, operator blocks, blocks of the prolongator
+ * from level l+1 (0 on the coarsest), and the lambda_max(D^-1 A) estimate. */
+int onsas_amg_level_info(onsas_ctx* ctx, int32_t level, int64_t* n, int32_t* bs, int64_t* nnzb,
+                         int64_t* nnz_p, double* lam);
+/* Level l >= 1 operator, 6x6 block CSR: row_ptr (n+1), col (nnzb), vals (36 per
+ * block, row-major; FP64 copy of the stored values).  NULL outputs are skipped. */
+int onsas_amg_get_operator(onsas_ctx* ctx, int32_t level, int32_t* row_ptr, int32_t* col,
+                           double* vals);
+/* Prolongator from level l+1 to level l, block CSR over level-l nodes with
+ * bs x 6 blocks: p_ptr (n+1), p_col (nnz_p), vals (bs*6 per block, row-major;
+ * FP64 copy of the stored FP16 (mixed) / FP32 (FP64 mode) values). */
+int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int32_t* p_col,
+                              double* vals);
 /* y = K x with the assembled tangent (n values each). */
 int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y);
 

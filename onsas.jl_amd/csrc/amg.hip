@@ -911,7 +911,7 @@ int amg_build(Ctx* c) {
   L0.n = nn;
   L0.nnzb = (int64_t)c->col.size();
   A.lev.push_back(L0);
-  // level-0 near-nullspace: translations now, rotations per aggregate below
+  // level-0 near-nullspace: translations now, rotations in the first pass below
   std::vector<double> Bcur(nn * 3 * 6, 0.0);
   for (int64_t i = 0; i < nn; ++i)
     for (int d = 0; d < 3; ++d) Bcur[i * 18 + d * 6 + d] = 1.0;
@@ -930,36 +930,37 @@ int amg_build(Ctx* c) {
       std::vector<int64_t> pos(aptr.begin(), aptr.end() - 1);
       for (int64_t i = 0; i < n; ++i) anodes[pos[agg[i]]++] = (int32_t)i;
     }
-    if (lev == 0) {  // rigid-body rotations about each aggregate's centroid
-      for (int64_t a = 0; a < na; ++a) {
-        double cx = 0, cy = 0, cz = 0;
-        const int64_t m = aptr[a + 1] - aptr[a];
-        for (int64_t k = aptr[a]; k < aptr[a + 1]; ++k) {
-          cx += c->X[anodes[k]];
-          cy += c->Y[anodes[k]];
-          cz += c->Z[anodes[k]];
-        }
-        cx /= m;
-        cy /= m;
-        cz /= m;
-        for (int64_t k = aptr[a]; k < aptr[a + 1]; ++k) {
-          const int64_t i = anodes[k];
-          const double x = c->X[i] - cx, y = c->Y[i] - cy, z = c->Z[i] - cz;
-          double* b = &Bcur[i * 18];
-          // rotations about x, y, z: columns 3, 4, 5 (rows: ux, uy, uz)
-          b[0 * 6 + 3] = 0;
-          b[1 * 6 + 3] = -z;
-          b[2 * 6 + 3] = y;
-          b[0 * 6 + 4] = z;
-          b[1 * 6 + 4] = 0;
-          b[2 * 6 + 4] = -x;
-          b[0 * 6 + 5] = -y;
-          b[1 * 6 + 5] = x;
-          b[2 * 6 + 5] = 0;
-          for (int d = 0; d < 3; ++d)
-            if ((c->fixmask[i] >> d) & 1u)
-              for (int q = 0; q < 6; ++q) b[d * 6 + q] = 0;
-        }
+    if (lev == 0) {
+      // rigid-body rotations about ONE point (the node centroid) for the whole
+      // mesh: each aggregate's R then holds its coordinates of the same global
+      // modes, which the next level's tentative prolongator needs (rotations
+      // about a per-aggregate point span the same local space, but their R
+      // would describe a different, non-rigid field in every aggregate)
+      double cx = 0, cy = 0, cz = 0;
+      for (int64_t i = 0; i < nn; ++i) {
+        cx += c->X[i];
+        cy += c->Y[i];
+        cz += c->Z[i];
+      }
+      cx /= (double)nn;
+      cy /= (double)nn;
+      cz /= (double)nn;
+      for (int64_t i = 0; i < nn; ++i) {
+        const double x = c->X[i] - cx, y = c->Y[i] - cy, z = c->Z[i] - cz;
+        double* b = &Bcur[i * 18];
+        // rotations about x, y, z: columns 3, 4, 5 (rows: ux, uy, uz)
+        b[0 * 6 + 3] = 0;
+        b[1 * 6 + 3] = -z;
+        b[2 * 6 + 3] = y;
+        b[0 * 6 + 4] = z;
+        b[1 * 6 + 4] = 0;
+        b[2 * 6 + 4] = -x;
+        b[0 * 6 + 5] = -y;
+        b[1 * 6 + 5] = x;
+        b[2 * 6 + 5] = 0;
+        for (int d = 0; d < 3; ++d)
+          if ((c->fixmask[i] >> d) & 1u)
+            for (int q = 0; q < 6; ++q) b[d * 6 + q] = 0;
       }
     }
     std::vector<double> T(n * bs * 6, 0.0), Bc(na * 36, 0.0);
@@ -1389,6 +1390,88 @@ int amg_prepare(Ctx* c) {
 double amg_fine_smoother(Ctx* c, void** d0) {
   *d0 = c->amg.lev[0].d_d0;
   return cheb(c->amg.lev[0].lam).inv_theta;
+}
+
+// ---- hierarchy inspection (tests, diagnostics) -------------------------------------
+
+// the hierarchy has values once a solve has run its numeric setup (lam_set)
+int amg_nlevels(Ctx* c) { return c->amg.built && c->amg.lam_set ? (int)c->amg.lev.size() : 0; }
+
+int amg_export_info(Ctx* c, int l, int64_t* n, int32_t* bs, int64_t* nnzb, int64_t* nnz_p,
+                    double* lam) {
+  const Amg& A = c->amg;
+  if (!A.built || !A.lam_set || l < 0 || l >= (int)A.lev.size())
+    return set_err(c, ONSAS_ERR_ARG, "no such multigrid level");
+  const AmgLevel& L = A.lev[l];
+  if (n) *n = L.n;
+  if (bs) *bs = L.bs;
+  if (nnzb) *nnzb = l == 0 ? c->nnzb : L.nnzb;
+  if (nnz_p) *nnz_p = l + 1 < (int)A.lev.size() ? L.nnz_p : 0;
+  if (lam) *lam = L.lam;
+  return ONSAS_OK;
+}
+
+namespace {
+template <typename T>
+int copy_back(Ctx* c, std::vector<T>& h, const void* d, size_t count) {
+  h.resize(count);
+  if (count)
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(h.data(), d, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+}  // namespace
+
+int amg_export_operator(Ctx* c, int l, int32_t* row_ptr, int32_t* col, double* vals) {
+  const Amg& A = c->amg;
+  if (!A.built || !A.lam_set || l < 1 || l >= (int)A.lev.size())
+    return set_err(c, ONSAS_ERR_ARG, "multigrid operators are exported for levels >= 1");
+  const AmgLevel& L = A.lev[l];
+  if (row_ptr) std::copy(L.row_ptr.begin(), L.row_ptr.end(), row_ptr);
+  if (col) std::copy(L.col.begin(), L.col.end(), col);
+  if (vals) {
+    const size_t m = 36 * (size_t)L.nnzb;
+    if (c->precision == ONSAS_PRECISION_FP64) {
+      std::vector<double> h;
+      ONSAS_TRY(copy_back(c, h, L.d_A, m));
+      std::copy(h.begin(), h.end(), vals);
+    } else {
+      std::vector<float> h;
+      ONSAS_TRY(copy_back(c, h, L.d_A, m));
+      std::copy(h.begin(), h.end(), vals);
+    }
+  }
+  return ONSAS_OK;
+}
+
+int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double* vals) {
+  const Amg& A = c->amg;
+  if (!A.built || !A.lam_set || l < 0 || l + 1 >= (int)A.lev.size())
+    return set_err(c, ONSAS_ERR_ARG, "no prolongator below this multigrid level");
+  const AmgLevel& L = A.lev[l];
+  if (p_ptr) {
+    std::vector<int32_t> h;
+    ONSAS_TRY(copy_back(c, h, L.d_p_ptr, L.n + 1));
+    std::copy(h.begin(), h.end(), p_ptr);
+  }
+  if (p_col) {
+    std::vector<int32_t> h;
+    ONSAS_TRY(copy_back(c, h, L.d_p_col, L.nnz_p));
+    std::copy(h.begin(), h.end(), p_col);
+  }
+  if (vals) {
+    const size_t m = (size_t)L.bs * 6 * L.nnz_p;
+    if (c->precision == ONSAS_PRECISION_FP64) {
+      std::vector<float> h;
+      ONSAS_TRY(copy_back(c, h, L.d_P, m));
+      std::copy(h.begin(), h.end(), vals);
+    } else {
+      std::vector<_Float16> h;
+      ONSAS_TRY(copy_back(c, h, L.d_P, m));
+      for (size_t k = 0; k < m; ++k) vals[k] = (double)(float)h[k];
+    }
+  }
+  return ONSAS_OK;
 }
 
 void amg_free(Ctx* c) {

@@ -521,6 +521,35 @@ int onsas_get_tangent(onsas_ctx* ctx, double* vals) {
   return ONSAS_OK;
 }
 
+int onsas_amg_levels(onsas_ctx* ctx, int32_t* nlevels) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!nlevels) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  *nlevels = onsas::amg_nlevels(c);
+  return ONSAS_OK;
+}
+
+int onsas_amg_level_info(onsas_ctx* ctx, int32_t level, int64_t* n, int32_t* bs, int64_t* nnzb,
+                         int64_t* nnz_p, double* lam) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  return onsas::amg_export_info(c, level, n, bs, nnzb, nnz_p, lam);
+}
+
+int onsas_amg_get_operator(onsas_ctx* ctx, int32_t level, int32_t* row_ptr, int32_t* col,
+                           double* vals) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  return onsas::amg_export_operator(c, level, row_ptr, col, vals);
+}
+
+int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int32_t* p_col,
+                              double* vals) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  return onsas::amg_export_prolongator(c, level, p_ptr, p_col, vals);
+}
+
 int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));

@@ -338,6 +338,11 @@ int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_re
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
 int amg_prepare(Ctx* c);  // refresh the smoother copy of the tangent (outside graphs)
 void amg_free(Ctx* c);
+int amg_nlevels(Ctx* c);  // hierarchy inspection (0 before it is built)
+int amg_export_info(Ctx* c, int l, int64_t* n, int32_t* bs, int64_t* nnzb, int64_t* nnz_p,
+                    double* lam);
+int amg_export_operator(Ctx* c, int l, int32_t* row_ptr, int32_t* col, double* vals);
+int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double* vals);
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2

@@ -34,8 +34,9 @@ EXPORTED = (
     "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_set_U", "onsas_get_fext",
     "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
     "onsas_tangent_matvec", "onsas_element_forces", "onsas_get_stress_strain",
-    "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points", "onsas_get_timings", "onsas_reset_timings",
-    "onsas_inject_error",
+    "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
+    "onsas_get_timings", "onsas_reset_timings", "onsas_inject_error", "onsas_amg_levels",
+    "onsas_amg_level_info", "onsas_amg_get_operator", "onsas_amg_get_prolongator",
 )
 
 
@@ -124,6 +125,10 @@ def load():
         "onsas_get_timings": ([_vp, C.POINTER(Timings)], C.c_int),
         "onsas_reset_timings": ([_vp], C.c_int),
         "onsas_inject_error": ([_vp, C.c_int32], C.c_int),
+        "onsas_amg_levels": ([_vp, _i32p], C.c_int),
+        "onsas_amg_level_info": ([_vp, C.c_int32, _i64p, _i32p, _i64p, _i64p, _dp], C.c_int),
+        "onsas_amg_get_operator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
+        "onsas_amg_get_prolongator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -281,6 +286,44 @@ class Context:
         col = np.empty(nnzb, dtype=np.int32)
         self._check(self.L.onsas_get_pattern(self.h, rp.ctypes.data, col.ctypes.data))
         return rp, col
+
+    # -- multigrid hierarchy (after a PCG solve with PRECOND_AMG)
+    def amg_levels(self):
+        n = C.c_int32()
+        self._check(self.L.onsas_amg_levels(self.h, C.byref(n)))
+        return n.value
+
+    def amg_level_info(self, level):
+        n, nnzb, nnzp = C.c_int64(), C.c_int64(), C.c_int64()
+        bs, lam = C.c_int32(), C.c_double()
+        self._check(self.L.onsas_amg_level_info(self.h, level, C.byref(n), C.byref(bs), C.byref(nnzb),
+                                                C.byref(nnzp), C.byref(lam)))
+        return dict(n=n.value, bs=bs.value, nnzb=nnzb.value, nnz_p=nnzp.value, lam=lam.value)
+
+    def amg_operator(self, level):
+        """Level >= 1 operator as a scipy BSR matrix (6x6 blocks, FP64 copy)."""
+        import scipy.sparse as sp
+        info = self.amg_level_info(level)
+        rp = np.empty(info["n"] + 1, dtype=np.int32)
+        col = np.empty(info["nnzb"], dtype=np.int32)
+        vals = np.empty(36 * info["nnzb"])
+        self._check(self.L.onsas_amg_get_operator(self.h, level, rp.ctypes.data, col.ctypes.data,
+                                                  vals.ctypes.data))
+        m = 6 * info["n"]
+        return sp.bsr_matrix((vals.reshape(-1, 6, 6), col, rp), shape=(m, m))
+
+    def amg_prolongator(self, level):
+        """Prolongator from level+1 to level as a scipy BSR matrix (bs x 6 blocks)."""
+        import scipy.sparse as sp
+        info = self.amg_level_info(level)
+        nc = self.amg_level_info(level + 1)["n"]
+        bs = info["bs"]
+        rp = np.empty(info["n"] + 1, dtype=np.int32)
+        col = np.empty(info["nnz_p"], dtype=np.int32)
+        vals = np.empty(bs * 6 * info["nnz_p"])
+        self._check(self.L.onsas_amg_get_prolongator(self.h, level, rp.ctypes.data, col.ctypes.data,
+                                                     vals.ctypes.data))
+        return sp.bsr_matrix((vals.reshape(-1, bs, 6), col, rp), shape=(bs * info["n"], 6 * nc))
 
     def tangent_bsr(self):
         """The assembled tangent as a scipy BSR matrix (3x3 blocks, FP64 copy)."""
