@@ -30,6 +30,7 @@
 // the PCG state says done, so replayed graphs cost nothing after convergence.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -561,27 +562,34 @@ __global__ __launch_bounds__(256) void k_bsr6_cheb(
   if (x_out) x_out[6 * I + l] = (x_base ? x_base[6 * I + l] : KT(0)) + vu + d;
 }
 
-// Large levels >= 1 on their sliced-ELL copy: one wave per slice of 8 rows, 8
-// lanes per row; every value load of a round reads 64 x 16 contiguous bytes.
-// After the reduction over the 8 lanes of a row, lanes ph < 6 finish component
-// ph of row s6_row[8 s + r] (rows sorted by length within windows; -1: none).
-template <typename KT, typename VT>
+// Large levels >= 1 on their sliced-ELL copy: one wave per slice of RPW rows,
+// 64 / RPW lanes per row (lane l serves row l % RPW); every value load of a
+// round reads 64 x 16 contiguous bytes.  The column indices of the next round
+// are fetched before this round's products, so the index -> x gather chain of
+// one round overlaps the value loads of the previous one.  After the reduction
+// over the lanes of a row, lanes l / RPW < 6 finish that component of row
+// s6_row[RPW s + l % RPW] (rows sorted by length within windows; -1: none).
+template <typename KT, typename VT, int RPW>
 __global__ __launch_bounds__(256) void k_sell6_cheb(
     const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
     const int32_t* __restrict__ s6_row, const VT* __restrict__ A6, const KT* __restrict__ v,
     const KT* b, KT* r_out,
     const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
     KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
+  static_assert(RPW == 2 || RPW == 4 || RPW == 8, "6 components need >= 8 lanes per row");
   if (pcg_stopped(st)) return;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (s >= nslices) return;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t q = s6_ptr[s]; q < s6_ptr[s + 1]; ++q) {
-    const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)s6_col[64 * (int64_t)q + l], 8);
+  const int32_t q0 = s6_ptr[s], q1 = s6_ptr[s + 1];
+  int32_t col = q0 < q1 ? s6_col[64 * (int64_t)q0 + l] : 0;
+  for (int32_t q = q0; q < q1; ++q) {
+    const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)col, 8);
     KT xv[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) xv[c] = xj[c];
+    col = q + 1 < q1 ? s6_col[64 * (int64_t)(q + 1) + l] : 0;
     const VT* a = A6 + 36 * 64 * (int64_t)q + 4 * l;
 #pragma unroll
     for (int ch = 0; ch < 9; ++ch) {
@@ -594,11 +602,11 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
     }
   }
 #pragma unroll
-  for (int o = 8; o < 64; o <<= 1)
+  for (int o = RPW; o < 64; o <<= 1)
 #pragma unroll
     for (int u = 0; u < 6; ++u) acc[u] += __shfl_xor(acc[u], o, 64);
-  const int64_t I = s6_row[8 * s + (l & 7)];
-  const int u = l >> 3;
+  const int64_t I = s6_row[RPW * s + (l & (RPW - 1))];
+  const int u = l / RPW;
   if (u >= 6 || I < 0) return;
   KT r[6];
 #pragma unroll
@@ -657,7 +665,11 @@ __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
 // entries.  With d0 non-null also d0[I] = s D_c^-1 b_c[I] (the first Chebyshev
 // step of the coarse level).
 // WPA waves per aggregate (1 or 4; with 4 the workgroup's waves add their sums
-// in wave order through LDS).
+// in wave order through LDS).  An FP32 lane takes two of its entries per pass
+// (lane stride apart) with clamped, unconditional index loads, so both row
+// gathers are in flight together (FP64: one, for the registers); a slot past
+// the end adds an exact zero, which keeps the lane's summation order (entries
+// l, l + 64 WPA, ...) of a one-entry loop.
 template <typename KT, int BS, int WPA>
 __global__ __launch_bounds__(256) void k_sa_restrict(
     const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_row,
@@ -665,22 +677,43 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
     const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc, const PcgState* st) {
   static_assert(WPA == 1 || WPA == 4, "one or four waves per aggregate");
   using PV = PT_t<KT>;
+  constexpr int kStride = 64 * WPA;
+  constexpr int kU = sizeof(KT) == 4 ? 2 : 1;
   __shared__ KT sh[4][6];
   if (pcg_stopped(st)) return;
-  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / (64 * WPA);
-  const int l = threadIdx.x % (64 * WPA);
+  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kStride;
+  const int l = threadIdx.x % kStride;
   if (WPA == 1 && I >= nc) return;  // WPA 4: one aggregate per workgroup, I < nc
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t e = rt_ptr[I] + l; e < rt_ptr[I + 1]; e += 64 * WPA) {
-    const int64_t i = rt_row[e];
-    const PV* pp = (const PV*)__builtin_assume_aligned(PT + (int64_t)BS * 6 * e, 2 * sizeof(PV));
-    KT rv[BS];
+  const int32_t e0 = rt_ptr[I], e1 = rt_ptr[I + 1];
+  for (int32_t e = e0 + l; e < e1; e += kU * kStride) {
+    int32_t ek[kU];
+    int64_t ik[kU];
 #pragma unroll
-    for (int a = 0; a < BS; ++a) rv[a] = rf[BS * i + a];
+    for (int k = 0; k < kU; ++k) {
+      ek[k] = min(e + k * kStride, e1 - 1);
+      ik[k] = rt_row[ek[k]];
+    }
+    PV pv[kU][BS * 6];
+    KT rv[kU][BS];
 #pragma unroll
-    for (int a = 0; a < BS; ++a)
+    for (int k = 0; k < kU; ++k) {  // unconditional loads (clamped addresses)
+      const PV* pp = (const PV*)__builtin_assume_aligned(PT + (int64_t)BS * 6 * ek[k], 2 * sizeof(PV));
 #pragma unroll
-      for (int u = 0; u < 6; ++u) acc[u] += pp[6 * a + u] * rv[a];
+      for (int q = 0; q < BS * 6; ++q) pv[k][q] = pp[q];
+#pragma unroll
+      for (int a = 0; a < BS; ++a) rv[k][a] = rf[BS * ik[k] + a];
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const bool ok = e + k * kStride < e1;
+#pragma unroll
+      for (int a = 0; a < BS; ++a) {
+        const KT ra = ok ? rv[k][a] : KT(0);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) acc[u] += (KT)pv[k][6 * a + u] * ra;
+      }
+    }
   }
   wave_sum6(acc);
   if (WPA == 4) {
@@ -703,23 +736,48 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
 }
 
 // x_f[i] += sum over the P blocks p of row i of P_p x_c[col(p)], one thread per
-// (node, dof).
+// (node, dof).  Eight blocks per pass (FP64: four, for the registers) with
+// clamped, unconditional index loads: the column indices, then all the pass's
+// coarse gathers, are in flight together (a row has about six blocks); a slot
+// past the row's end adds an exact zero, so the sum runs in block order as in a
+// one-block loop.
 template <typename KT, int BS>
 __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
                              const PT_t<KT>* __restrict__ P, const KT* __restrict__ xc,
                              KT* __restrict__ xf, int64_t nf, const PcgState* st) {
   using PV = PT_t<KT>;
+  constexpr int kPass = sizeof(KT) == 4 ? 8 : 2;
   if (pcg_stopped(st)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= BS * nf) return;
   const int64_t i = t / BS;
   const int a = (int)(t % BS);
+  const int32_t p0 = p_ptr[i], p1 = p_ptr[i + 1];
   KT acc = 0;
-  for (int32_t p = p_ptr[i]; p < p_ptr[i + 1]; ++p) {
-    const PV* pp = (const PV*)__builtin_assume_aligned(P + (int64_t)BS * 6 * p + 6 * a, 2 * sizeof(PV));
-    const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)p_col[p], 8);
+  for (int32_t p = p0; p < p1; p += kPass) {
+    int32_t cj[kPass];
 #pragma unroll
-    for (int u = 0; u < 6; ++u) acc += pp[u] * xj[u];
+    for (int k = 0; k < kPass; ++k) cj[k] = p_col[min(p + k, p1 - 1)];
+    PV pv[kPass][6];
+    KT xv[kPass][6];
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {  // unconditional loads (clamped addresses)
+      const PV* pp = (const PV*)__builtin_assume_aligned(
+          P + (int64_t)BS * 6 * min(p + k, p1 - 1) + 6 * a, 2 * sizeof(PV));
+      const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)cj[k], 8);
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        pv[k][u] = pp[u];
+        xv[k][u] = xj[u];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every gather issued before the first product
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const bool ok = p + k < p1;
+#pragma unroll
+      for (int u = 0; u < 6; ++u) acc += (KT)pv[k][u] * (ok ? xv[k][u] : KT(0));
+    }
   }
   xf[t] += acc;
 }
@@ -803,6 +861,18 @@ template <typename KT>
 int upload_kt(Ctx* c, void** d, const std::vector<double>& h) {
   std::vector<KT> v(h.begin(), h.end());
   return upload(c, (KT**)d, v);
+}
+
+// Rows per sliced-ELL slice of a coarse level: fewer rows per wave split a row
+// over more lanes (fewer dependent index -> gather rounds, more waves in flight)
+// at the price of more padding; ONSAS_SELL6_RPW (8, 4 or 2) overrides.
+int sell6_rows_per_wave(int64_t rows) {
+  if (const char* e = std::getenv("ONSAS_SELL6_RPW")) {
+    const int r = std::atoi(e);
+    if (r == 2 || r == 4 || r == 8) return r;
+  }
+  (void)rows;
+  return 8;
 }
 
 // greedy aggregation of a node graph (CSR, self included)
@@ -1039,7 +1109,7 @@ int amg_build(Ctx* c) {
     }
     Lc.nnzb = (int64_t)Lc.col.size();
     // sliced-ELL layout of a large coarse operator (not the coarsest level):
-    // rows ordered by decreasing length within windows, 8 per slice
+    // rows ordered by decreasing length within windows, rpw per slice
     std::vector<int32_t> s6_ptr(1, 0), s6_col, s6_src, s6_row;
     if (na > kMaxCoarseNodes && na >= kSell6MinRows) {
       std::vector<int32_t> order(na);
@@ -1048,17 +1118,20 @@ int amg_build(Ctx* c) {
       for (int64_t w0 = 0; w0 < na; w0 += kSell6Window)
         std::stable_sort(order.begin() + w0, order.begin() + std::min<int64_t>(w0 + kSell6Window, na),
                          [&](int32_t x, int32_t y) { return len(x) > len(y); });
-      for (int64_t r0 = 0; r0 < na; r0 += 8) {
+      const int rpw = sell6_rows_per_wave(na);
+      const int lpr = 64 / rpw;
+      Lc.rpw6 = rpw;
+      for (int64_t r0 = 0; r0 < na; r0 += rpw) {
         int32_t width = 0;
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < rpw; ++r) {
           const int32_t I = r0 + r < na ? order[r0 + r] : -1;
           s6_row.push_back(I);
-          if (I >= 0) width = std::max(width, (len(I) + 7) / 8);
+          if (I >= 0) width = std::max(width, (len(I) + lpr - 1) / lpr);
         }
         for (int32_t m = 0; m < width; ++m)
           for (int l = 0; l < 64; ++l) {
-            const int32_t I = s6_row[r0 + (l & 7)];
-            const int32_t slot = 8 * m + (l >> 3);
+            const int32_t I = s6_row[r0 + (l & (rpw - 1))];
+            const int32_t slot = lpr * m + l / rpw;
             const bool ok = I >= 0 && slot < len(I);
             const int32_t k = ok ? Lc.row_ptr[I] + slot : -1;
             s6_src.push_back(k);
@@ -1140,6 +1213,28 @@ int amg_build(Ctx* c) {
 
 namespace {
 
+template <typename KT, typename VT, int R>
+void launch_sell6_r(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
+                    double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
+                    const PcgState* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
+                     L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const VT*)A6, v, b, r_out,
+                     (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6, L.n,
+                     st);
+}
+
+template <typename KT, typename VT>
+void launch_sell6(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
+                  double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
+                  const PcgState* st, hipStream_t s) {
+  if (L.rpw6 == 2)
+    launch_sell6_r<KT, VT, 2>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
+  else if (L.rpw6 == 4)
+    launch_sell6_r<KT, VT, 4>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
+  else
+    launch_sell6_r<KT, VT, 8>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
+}
+
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
 // approx: the product's result only feeds a smoothing update; it may use the
 // smoother copy of the operator (fine level in FP64 mode, sliced-ELL coarse
@@ -1172,15 +1267,9 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
     const AmgLevel& L = c->amg.lev[l];
     using CT = typename CoarseSmootherType<KT>::type;
     if (L.nslices6 > 0 && approx)
-      hipLaunchKernelGGL((k_sell6_cheb<KT, CT>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
-                         L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const CT*)L.d_A6s, v, b, r_out,
-                         (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6,
-                         L.n, st);
+      launch_sell6<KT, CT>(L, L.d_A6s, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
     else if (L.nslices6 > 0)
-      hipLaunchKernelGGL((k_sell6_cheb<KT, KT>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
-                         L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const KT*)L.d_A6, v, b, r_out,
-                         (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6,
-                         L.n, st);
+      launch_sell6<KT, KT>(L, L.d_A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
     else
       hipLaunchKernelGGL(k_bsr6_cheb<KT>, dim3(nblocks(L.n, 4)), dim3(256), 0, s, L.d_row_ptr,
                          L.d_col, (const KT*)L.d_A, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1,

@@ -121,13 +121,14 @@ struct AmgLevel {
   void* d_A = nullptr;                // nnzb x 36 KT (levels >= 1)
   void* d_Dinv = nullptr;             // n x 36 KT (levels >= 1)
   // sliced-ELL copy of A for the cycle (large levels >= 1 but the coarsest):
-  // rows ordered by decreasing length within windows, slices of 8 of them, one
-  // wave each; lane l = 8 ph + r serves row s6_row[8 s + r] (-1: none), slot
-  // 8 m + ph of round m.  Round q of the level (s6_ptr[s] <= q < s6_ptr[s+1])
+  // rows ordered by decreasing length within windows, slices of R = rpw6 of
+  // them, one wave each; lane l = R ph + r serves row s6_row[R s + r] (-1:
+  // none), slot (64 / R) m + ph of round m.  Round q of the level (s6_ptr[s] <= q < s6_ptr[s+1])
   // holds lane l's block column at s6_col[64 q + l] and its 36 values in 9
   // chunks of 4 at A6[((9 q + chunk) 64 + l) 4 + j]; s6_src is the source block
   // in A (-1: padding, zero values).
   int64_t nslices6 = 0, nrounds6 = 0;
+  int rpw6 = 8;                       // rows per slice: 8, 4 or 2
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
   void* d_A6 = nullptr;
   void* d_A6s = nullptr;              // its smoothing-only copy (BF16 mixed, FP32 FP64)
