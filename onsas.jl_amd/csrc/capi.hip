@@ -40,6 +40,24 @@ int check_ready(Ctx* c) {
   return ONSAS_OK;
 }
 
+// State vectors (3 nn doubles) cross PCIe through the pinned stage: one DMA at
+// full link rate instead of the runtime's chunked pageable copy.
+int download_state(Ctx* c, double* dst, const double* d_src) {
+  const size_t bytes = 3 * (size_t)c->nn * sizeof(double);
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->h_stage, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::memcpy(dst, c->h_stage, bytes);
+  return ONSAS_OK;
+}
+
+int upload_state(Ctx* c, double* d_dst, const double* src) {
+  const size_t bytes = 3 * (size_t)c->nn * sizeof(double);
+  std::memcpy(c->h_stage, src, bytes);
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(d_dst, c->h_stage, bytes, hipMemcpyHostToDevice, c->stream));
+  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return ONSAS_OK;
+}
+
 int check_building(Ctx* c) {
   if (!c) return ONSAS_ERR_ARG;
   if (c->injected) {
@@ -241,6 +259,7 @@ void onsas_destroy(onsas_ctx* ctx) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->ev_spmv)
@@ -376,8 +395,10 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
   if (npressure != (int32_t)c->pressure_faces.size() || (npressure > 0 && !pressures))
     return set_err(c, ONSAS_ERR_ARG, "onsas_set_loads: one pressure value per pressure group expected");
   const size_t n = 3 * (size_t)c->nn;
-  if (nodal)
-    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_nodal, nodal, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (nodal) {
+    std::memcpy(c->h_stage, nodal, n * sizeof(double));
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_nodal, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
   else
     ONSAS_HIP_TRY(c, hipMemsetAsync(c->d_nodal, 0, n * sizeof(double), c->stream));
   if (npressure > 0)
@@ -455,8 +476,7 @@ int onsas_get_U(onsas_ctx* ctx, double* U) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL output");
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(U, c->d_U, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  ONSAS_TRY(download_state(c, U, c->d_U));
   return ONSAS_OK;
 }
 
@@ -464,8 +484,7 @@ int onsas_set_U(onsas_ctx* ctx, const double* U) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL input");
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_U, U, 3 * c->nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  ONSAS_TRY(upload_state(c, c->d_U, U));
   c->assembled = false;
   return ONSAS_OK;
 }
@@ -474,8 +493,7 @@ int onsas_get_fext(onsas_ctx* ctx, double* f) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (!f) return set_err(c, ONSAS_ERR_ARG, "NULL output");
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(f, c->d_fext, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  ONSAS_TRY(download_state(c, f, c->d_fext));
   return ONSAS_OK;
 }
 
@@ -483,8 +501,7 @@ int onsas_get_fint(onsas_ctx* ctx, double* f) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (!f) return set_err(c, ONSAS_ERR_ARG, "NULL output");
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(f, c->d_fint, 3 * c->nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  ONSAS_TRY(download_state(c, f, c->d_fint));
   return ONSAS_OK;
 }
 

@@ -265,6 +265,7 @@ struct Ctx {
   double* d_part = nullptr;  // partial-sum arrays, kPartArrays x kMaxPartials
   Scalars* d_sc = nullptr;
   Scalars* h_sc = nullptr;   // pinned
+  double* h_stage = nullptr;  // pinned, 3 nn values: staging of state vector copies
   // pressure faces
   int32_t *d_face_conn = nullptr, *d_face_grp = nullptr;  // all faces, 3*nf and nf
   int32_t *d_nf_ptr = nullptr, *d_nf_idx = nullptr;       // node -> faces CSR

@@ -409,6 +409,7 @@ int finalize_model(Ctx* c) {
   ONSAS_TRY(dev_alloc(c, (void**)&c->d_part, sizeof(double) * kPartArrays * kMaxPartials));
   ONSAS_TRY(dev_alloc(c, (void**)&c->d_sc, sizeof(Scalars)));
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
+  ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_stage, sizeof(double) * n, hipHostMallocDefault));
 
   // reduction grid: one row per thread when it fits, whole 64-row slices per
   // workgroup, a multiple of 8 workgroups (XCD mapping), at most kMaxPartials

@@ -113,7 +113,9 @@ class GpuStaticState:
     def apply_loads(self, t):
         """external_forces .= 0; apply!(sa, load_bcs) at time t
         (NonLinearStaticAnalyses.jl:86-87, StructuralAnalyses.jl:228-241)."""
-        nodal = self.s.bcs.nodal_loads(t, self.s.mesh.coords, self.n)
+        # no GlobalLoad: the engine clears its nodal loads itself (no 3n upload)
+        nodal = (self.s.bcs.nodal_loads(t, self.s.mesh.coords, self.n)
+                 if self.s.bcs.has_nodal_loads() else None)
         pressures = [bc(t) for bc in self.pressure_bcs]
         self.ctx.set_loads(nodal, pressures)
 

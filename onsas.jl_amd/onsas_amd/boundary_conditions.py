@@ -137,6 +137,10 @@ class StructuralBoundaryCondition:
         _, first = np.unique(allf, return_index=True)
         return allf[np.sort(first)]
 
+    def has_nodal_loads(self):
+        """True if any GlobalLoad is declared (else the nodal load vector is zero)."""
+        return any(isinstance(bc, GlobalLoad) for bc, _ in self._pairs)
+
     def nodal_loads(self, t, coords, n):
         """Sum of every GlobalLoad at time t as a dense n-vector, duplicate dofs
         summed (StructuralBoundaryConditions.jl:195-220)."""
