@@ -477,12 +477,30 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
     const int32_t* __restrict__ scol, const VT* __restrict__ sval, const KT* __restrict__ v,
     const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out,
     const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st) {
-  if (pcg_stopped(st)) return;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (s >= nslices) return;
-  const int32_t b0 = sell_ptr[s];
-  const int32_t rounds = slice_rounds[s];
+  // the epilogue's row operands depend on nothing but the row: loaded first,
+  // with the stop flag, so their latency hides under the product's
+  const int64_t i = kSellRows * s + lane % kSellRows;
+  const int64_t ic = i < nn ? i : nn - 1;
+  KT bv[3] = {0, 0, 0}, vv3[3] = {0, 0, 0}, xb[3] = {0, 0, 0}, dm[9];
+  if (b)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) bv[q] = b[3 * ic + q];
+  if (d_out || x_out) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dm[q] = Dinv[9 * ic + q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) vv3[q] = v[3 * ic + q];
+    if (x_base)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) xb[q] = x_base[3 * ic + q];
+  }
+  const int32_t su = __builtin_amdgcn_readfirstlane((int32_t)s);  // wave-uniform: scalar loads
+  const int32_t b0 = sell_ptr[su];
+  const int32_t rounds = slice_rounds[su];
+  if (pcg_stopped(st)) return;
   const uint32_t rs = (uint32_t)rstride;
   const int32_t* cp = scol + b0 + lane;
   const VT* vp = sval + 9u * (uint32_t)b0 + lane;
@@ -501,23 +519,21 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
     a1 += __shfl_xor(a1, o, 64);
     a2 += __shfl_xor(a2, o, 64);
   }
-  const int64_t i = kSellRows * s + lane % kSellRows;
   if (lane >= kSellRows || i >= nn) return;
   KT r[3] = {-a0, -a1, -a2};
   if (b)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) r[q] += b[3 * i + q];
+    for (int q = 0; q < 3; ++q) r[q] += bv[q];
   if (r_out)
 #pragma unroll
     for (int q = 0; q < 3; ++q) r_out[3 * i + q] = r[q];
   if (!d_out && !x_out) return;
-  const KT* dm = Dinv + 9 * i;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const KT vq = v[3 * i + q];
+    const KT vq = vv3[q];
     const KT d = c1 * vq + c2 * (dm[3 * q] * r[0] + dm[3 * q + 1] * r[1] + dm[3 * q + 2] * r[2]);
     if (d_out) d_out[3 * i + q] = d;
-    if (x_out) x_out[3 * i + q] = (x_base ? x_base[3 * i + q] : KT(0)) + vq + d;
+    if (x_out) x_out[3 * i + q] = xb[q] + vq + d;
   }
 }
 
@@ -577,12 +593,28 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
     const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
     KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
   static_assert(RPW == 2 || RPW == 4 || RPW == 8, "6 components need >= 8 lanes per row");
-  if (pcg_stopped(st)) return;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (s >= nslices) return;
+  // the epilogue's row operands first (see k_sell_cheb)
+  const int64_t I = s6_row[RPW * s + (l & (RPW - 1))];
+  const int u = l / RPW;
+  const int64_t Ic = I >= 0 ? I : 0;
+  const int uc = u < 6 ? u : 0;
+  KT bv[6] = {0, 0, 0, 0, 0, 0}, dm[6], vu = 0, xb = 0;
+  if (b)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) bv[m] = b[6 * Ic + m];
+  if (d_out || x_out) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) dm[m] = Dinv[36 * Ic + 6 * uc + m];
+    vu = v[6 * Ic + uc];
+    if (x_base) xb = x_base[6 * Ic + uc];
+  }
+  const int32_t su = __builtin_amdgcn_readfirstlane((int32_t)s);  // wave-uniform: scalar loads
+  const int32_t q0 = s6_ptr[su], q1 = s6_ptr[su + 1];
+  if (pcg_stopped(st)) return;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  const int32_t q0 = s6_ptr[s], q1 = s6_ptr[s + 1];
   int32_t col = q0 < q1 ? s6_col[64 * (int64_t)q0 + l] : 0;
   for (int32_t q = q0; q < q1; ++q) {
     const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)col, 8);
@@ -604,24 +636,20 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
 #pragma unroll
   for (int o = RPW; o < 64; o <<= 1)
 #pragma unroll
-    for (int u = 0; u < 6; ++u) acc[u] += __shfl_xor(acc[u], o, 64);
-  const int64_t I = s6_row[RPW * s + (l & (RPW - 1))];
-  const int u = l / RPW;
+    for (int w = 0; w < 6; ++w) acc[w] += __shfl_xor(acc[w], o, 64);
   if (u >= 6 || I < 0) return;
   KT r[6];
 #pragma unroll
-  for (int m = 0; m < 6; ++m) r[m] = (b ? b[6 * I + m] : KT(0)) - acc[m];
+  for (int m = 0; m < 6; ++m) r[m] = bv[m] - acc[m];
   const KT ru = pick6(r, u);
   if (r_out) r_out[6 * I + u] = ru;
   if (!d_out && !x_out) return;
-  const KT* dm = Dinv + 36 * I + 6 * u;
   KT dr = 0;
 #pragma unroll
   for (int m = 0; m < 6; ++m) dr += dm[m] * r[m];
-  const KT vu = v[6 * I + u];
   const KT d = c1 * vu + c2 * dr;
   if (d_out) d_out[6 * I + u] = d;
-  if (x_out) x_out[6 * I + u] = (x_base ? x_base[6 * I + u] : KT(0)) + vu + d;
+  if (x_out) x_out[6 * I + u] = xb + vu + d;
 }
 
 // Block-CSR -> sliced-ELL values: one thread per (round slot, value).
@@ -680,12 +708,19 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
   constexpr int kStride = 64 * WPA;
   constexpr int kU = sizeof(KT) == 4 ? 2 : 1;
   __shared__ KT sh[4][6];
-  if (pcg_stopped(st)) return;
   const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kStride;
   const int l = threadIdx.x % kStride;
   if (WPA == 1 && I >= nc) return;  // WPA 4: one aggregate per workgroup, I < nc
+  // the coarse row's D^-1 first (its latency hides under the gathers); the
+  // aggregate index is uniform over the wave, so its bounds are scalar loads
+  KT dm[6];
+  if (d0)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) dm[m] = Dinv_c[36 * I + 6 * (l < 6 ? l : 0) + m];
+  const int32_t Iu = __builtin_amdgcn_readfirstlane((int32_t)I);
+  const int32_t e0 = rt_ptr[Iu], e1 = rt_ptr[Iu + 1];
+  if (pcg_stopped(st)) return;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  const int32_t e0 = rt_ptr[I], e1 = rt_ptr[I + 1];
   for (int32_t e = e0 + l; e < e1; e += kU * kStride) {
     int32_t ek[kU];
     int64_t ik[kU];
@@ -727,7 +762,6 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
   if (l >= 6) return;
   bc[6 * I + l] = pick6(acc, l);
   if (d0) {
-    const KT* dm = Dinv_c + 36 * I + 6 * l;
     KT dr = 0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) dr += dm[m] * acc[m];
@@ -747,12 +781,13 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
                              KT* __restrict__ xf, int64_t nf, const PcgState* st) {
   using PV = PT_t<KT>;
   constexpr int kPass = sizeof(KT) == 4 ? 8 : 2;
-  if (pcg_stopped(st)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= BS * nf) return;
   const int64_t i = t / BS;
   const int a = (int)(t % BS);
+  const KT x0 = xf[t];  // loaded with the row's bounds, ahead of the gathers
   const int32_t p0 = p_ptr[i], p1 = p_ptr[i + 1];
+  if (pcg_stopped(st)) return;
   KT acc = 0;
   for (int32_t p = p0; p < p1; p += kPass) {
     int32_t cj[kPass];
@@ -779,7 +814,7 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
       for (int u = 0; u < 6; ++u) acc += (KT)pv[k][u] * (ok ? xv[k][u] : KT(0));
     }
   }
-  xf[t] += acc;
+  xf[t] = x0 + acc;
 }
 
 // x = Ainv b on the coarsest level (dense): one wave per row, lanes over its
