@@ -689,44 +689,50 @@ __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
 }
 
 // b_c[I] = sum over the P blocks of column I (entries rt_ptr[I].., row order; PT
-// holds them contiguously) of P_e^T r_row(e): a wave per aggregate, lanes over its
-// entries.  With d0 non-null also d0[I] = s D_c^-1 b_c[I] (the first Chebyshev
-// step of the coarse level).
-// WPA waves per aggregate (1 or 4; with 4 the workgroup's waves add their sums
-// in wave order through LDS).  An FP32 lane takes two of its entries per pass
-// (lane stride apart) with clamped, unconditional index loads, so both row
-// gathers are in flight together (FP64: one, for the registers); a slot past
-// the end adds an exact zero, which keeps the lane's summation order (entries
-// l, l + 64 WPA, ...) of a one-entry loop.
-template <typename KT, int BS, int WPA>
+// holds them contiguously) of P_e^T r_row(e), LPA lanes per aggregate over its
+// entries: 32 (two aggregates per wave, half the waves: the fine level, about
+// a hundred entries each), 64, or 256 (the workgroup's four waves add their
+// sums in wave order through LDS).  With d0 non-null also d0[I] = s D_c^-1
+// b_c[I] (the first Chebyshev step of the coarse level).  An FP32 lane takes
+// two of its entries per pass (LPA apart) with clamped, unconditional index
+// loads, so both row gathers are in flight together (FP64: one, for the
+// registers); a slot past the end adds an exact zero, so the lane's sum keeps
+// the order l, l + LPA, ... of a one-entry loop.
+template <typename KT, int BS, int LPA>
 __global__ __launch_bounds__(256) void k_sa_restrict(
     const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_row,
     const PT_t<KT>* __restrict__ PT, const KT* __restrict__ rf, KT* __restrict__ bc,
     const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc, const PcgState* st) {
-  static_assert(WPA == 1 || WPA == 4, "one or four waves per aggregate");
+  static_assert(LPA == 32 || LPA == 64 || LPA == 256, "32, 64 or 256 lanes per aggregate");
   using PV = PT_t<KT>;
-  constexpr int kStride = 64 * WPA;
   constexpr int kU = sizeof(KT) == 4 ? 2 : 1;
   __shared__ KT sh[4][6];
-  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kStride;
-  const int l = threadIdx.x % kStride;
-  if (WPA == 1 && I >= nc) return;  // WPA 4: one aggregate per workgroup, I < nc
-  // the coarse row's D^-1 first (its latency hides under the gathers); the
-  // aggregate index is uniform over the wave, so its bounds are scalar loads
+  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / LPA;
+  const int l = threadIdx.x % LPA;
+  if (LPA < 256 && I >= nc) return;  // LPA 256: one aggregate per workgroup, I < nc
+  // the coarse row's D^-1 first (its latency hides under the gathers); an
+  // aggregate index uniform over the wave makes its bounds scalar loads
   KT dm[6];
   if (d0)
 #pragma unroll
     for (int m = 0; m < 6; ++m) dm[m] = Dinv_c[36 * I + 6 * (l < 6 ? l : 0) + m];
-  const int32_t Iu = __builtin_amdgcn_readfirstlane((int32_t)I);
-  const int32_t e0 = rt_ptr[Iu], e1 = rt_ptr[Iu + 1];
+  int32_t e0, e1;
+  if (LPA >= 64) {
+    const int32_t Iu = __builtin_amdgcn_readfirstlane((int32_t)I);
+    e0 = rt_ptr[Iu];
+    e1 = rt_ptr[Iu + 1];
+  } else {
+    e0 = rt_ptr[I];
+    e1 = rt_ptr[I + 1];
+  }
   if (pcg_stopped(st)) return;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t e = e0 + l; e < e1; e += kU * kStride) {
+  for (int32_t e = e0 + l; e < e1; e += kU * LPA) {
     int32_t ek[kU];
     int64_t ik[kU];
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
-      ek[k] = min(e + k * kStride, e1 - 1);
+      ek[k] = min(e + k * LPA, e1 - 1);
       ik[k] = rt_row[ek[k]];
     }
     PV pv[kU][BS * 6];
@@ -741,7 +747,7 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
     }
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
-      const bool ok = e + k * kStride < e1;
+      const bool ok = e + k * LPA < e1;
 #pragma unroll
       for (int a = 0; a < BS; ++a) {
         const KT ra = ok ? rv[k][a] : KT(0);
@@ -750,8 +756,12 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
       }
     }
   }
-  wave_sum6(acc);
-  if (WPA == 4) {
+  // fixed tree over the aggregate's lanes (within its half of the wave for 32)
+#pragma unroll
+  for (int o = (LPA < 64 ? LPA : 64) / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
+  if (LPA == 256) {
     if ((threadIdx.x & 63) < 6) sh[threadIdx.x >> 6][threadIdx.x & 63] = pick6(acc, threadIdx.x & 63);
     __syncthreads();
     if (l < 6) {
@@ -815,6 +825,56 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
     }
   }
   xf[t] = x0 + acc;
+}
+
+// The same product with one thread per node (all BS dofs): each coarse block
+// of x is gathered once for the node's BS rows, and a third as many waves (the
+// fine level, BS = 3) run in one generation.  kPass blocks per pass as above;
+// every dof's sum keeps the block order of the one-thread-per-dof kernel.
+template <typename KT, int BS, int kPass>
+__global__ __launch_bounds__(256) void k_sa_prolong_node(
+    const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
+    const PT_t<KT>* __restrict__ P, const KT* __restrict__ xc, KT* __restrict__ xf, int64_t nf,
+    const PcgState* st) {
+  using PV = PT_t<KT>;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  KT x0[BS];
+#pragma unroll
+  for (int a = 0; a < BS; ++a) x0[a] = xf[BS * i + a];
+  const int32_t p0 = p_ptr[i], p1 = p_ptr[i + 1];
+  if (pcg_stopped(st)) return;
+  KT acc[BS];
+#pragma unroll
+  for (int a = 0; a < BS; ++a) acc[a] = KT(0);
+  for (int32_t p = p0; p < p1; p += kPass) {
+    int32_t cj[kPass];
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) cj[k] = p_col[min(p + k, p1 - 1)];
+    PV pv[kPass][BS * 6];
+    KT xv[kPass][6];
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {  // unconditional loads (clamped addresses)
+      const PV* pp = (const PV*)__builtin_assume_aligned(P + (int64_t)BS * 6 * min(p + k, p1 - 1),
+                                                         2 * sizeof(PV));
+      const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)cj[k], 8);
+#pragma unroll
+      for (int q = 0; q < BS * 6; ++q) pv[k][q] = pp[q];
+#pragma unroll
+      for (int u = 0; u < 6; ++u) xv[k][u] = xj[u];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every gather issued before the first product
+#pragma unroll
+    for (int k = 0; k < kPass; ++k) {
+      const bool ok = p + k < p1;
+#pragma unroll
+      for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int u = 0; u < 6; ++u) acc[a] += (KT)pv[k][6 * a + u] * (ok ? xv[k][u] : KT(0));
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < BS; ++a) xf[BS * i + a] = x0[a] + acc[a];
 }
 
 // x = Ainv b on the coarsest level (dense): one wave per row, lanes over its
@@ -1415,11 +1475,11 @@ template <typename KT, int BS>
 void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, KT sc, KT* d0c,
                      const PcgState* st) {
   if (L.restrict_wide)
-    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 4>), dim3((unsigned)C.n), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 256>), dim3((unsigned)C.n), dim3(256), 0, c->stream,
                        L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
   else
-    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 1>), dim3(nblocks(C.n, 4)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 32>), dim3(nblocks(C.n, 8)), dim3(256), 0, c->stream,
                        L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
 }
@@ -1472,8 +1532,9 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     KT* d0 = (KT*)L.d_d0;
     KT* z = l == 0 ? z0 : x;
     if (l == 0)
-      hipLaunchKernelGGL((k_sa_prolong<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
-                         L.d_p_ptr, L.d_p_col, (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
+      hipLaunchKernelGGL((k_sa_prolong_node<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
+                         dim3(nblocks(L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
+                         (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
                          L.d_p_ptr, L.d_p_col, (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
