@@ -205,6 +205,16 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out);
 
 /* ---- state access -------------------------------------------------------- */
 int onsas_get_U(onsas_ctx* ctx, double* U);       /* displacements(state), n values */
+/* The same copy in two halves, so that a load step's store! (src/StructuralSolvers/
+ * Solutions.jl, the per-step copy of U) overlaps the next step's work:
+ * onsas_snapshot_U enqueues the copy of U, as of the work enqueued so far,
+ * into pinned snapshot buffer `slot` (0 or 1) and returns at once;
+ * onsas_read_snapshot waits for that copy and writes the n values to U.  The
+ * read uses no stream and touches nothing but the slot, so it may run on
+ * another thread while this thread keeps driving the context; a slot must be
+ * read before it is snapshot again. */
+int onsas_snapshot_U(onsas_ctx* ctx, int32_t slot);
+int onsas_read_snapshot(onsas_ctx* ctx, int32_t slot, double* U);
 int onsas_set_U(onsas_ctx* ctx, const double* U); /* initial_state / resume         */
 int onsas_get_fext(onsas_ctx* ctx, double* fext); /* external_forces(state)         */
 int onsas_get_fint(onsas_ctx* ctx, double* fint); /* internal_forces(state)         */

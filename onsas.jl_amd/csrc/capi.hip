@@ -260,6 +260,10 @@ void onsas_destroy(onsas_ctx* ctx) {
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (double* h : c->h_snap)
+    if (h) (void)hipHostFree(h);
+  for (auto& e : c->ev_snap)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->ev_spmv)
@@ -477,6 +481,26 @@ int onsas_get_U(onsas_ctx* ctx, double* U) {
   ONSAS_TRY(check_ready(c));
   if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL output");
   ONSAS_TRY(download_state(c, U, c->d_U));
+  return ONSAS_OK;
+}
+
+int onsas_snapshot_U(onsas_ctx* ctx, int32_t slot) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (slot < 0 || slot > 1) return set_err(c, ONSAS_ERR_ARG, "snapshot slot must be 0 or 1");
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->h_snap[slot], c->d_U, 3 * c->nn * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev_snap[slot], c->stream));
+  return ONSAS_OK;
+}
+
+int onsas_read_snapshot(onsas_ctx* ctx, int32_t slot, double* U) {
+  Ctx* c = C(ctx);
+  if (!c || !c->finalized || slot < 0 || slot > 1 || !U) return ONSAS_ERR_ARG;
+  // no stream work and no shared state but the slot: safe beside the
+  // context's other calls on another thread
+  if (hipEventSynchronize(c->ev_snap[slot]) != hipSuccess) return ONSAS_ERR_HIP;
+  std::memcpy(U, c->h_snap[slot], 3 * c->nn * sizeof(double));
   return ONSAS_OK;
 }
 

@@ -266,6 +266,8 @@ struct Ctx {
   Scalars* d_sc = nullptr;
   Scalars* h_sc = nullptr;   // pinned
   double* h_stage = nullptr;  // pinned, 3 nn values: staging of state vector copies
+  double* h_snap[2] = {};     // pinned, 3 nn values each: U snapshots (onsas_snapshot_U)
+  hipEvent_t ev_snap[2] = {};  // ... their copies' completion
   // pressure faces
   int32_t *d_face_conn = nullptr, *d_face_grp = nullptr;  // all faces, 3*nf and nf
   int32_t *d_nf_ptr = nullptr, *d_nf_idx = nullptr;       // node -> faces CSR

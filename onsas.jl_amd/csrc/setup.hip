@@ -410,6 +410,8 @@ int finalize_model(Ctx* c) {
   ONSAS_TRY(dev_alloc(c, (void**)&c->d_sc, sizeof(Scalars)));
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_stage, sizeof(double) * n, hipHostMallocDefault));
+  for (double*& h : c->h_snap)
+    ONSAS_HIP_TRY(c, hipHostMalloc((void**)&h, sizeof(double) * n, hipHostMallocDefault));
 
   // reduction grid: one row per thread when it fits, whole 64-row slices per
   // workgroup, a multiple of 8 workgroups (XCD mapping), at most kMaxPartials
@@ -466,6 +468,7 @@ int finalize_model(Ctx* c) {
 
   if (c->gather) ONSAS_TRY(launch_gather_init(c));
   for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
+  for (auto& e : c->ev_snap) ONSAS_HIP_TRY(c, hipEventCreate(&e));
   // timing-only events (read after a stream synchronize): no system-scope fence,
   // so the stamped dispatch pays no cache writeback and invalidation
   for (auto& e : c->ev_spmv)

@@ -31,7 +31,8 @@ EXPORTED = (
     "onsas_create", "onsas_destroy", "onsas_last_error", "onsas_version", "onsas_set_nodes",
     "onsas_add_tet_group", "onsas_add_truss_group", "onsas_add_pressure_faces",
     "onsas_set_fixed_dofs", "onsas_finalize", "onsas_set_linear_solver", "onsas_set_loads",
-    "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_set_U", "onsas_get_fext",
+    "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_snapshot_U",
+    "onsas_read_snapshot", "onsas_set_U", "onsas_get_fext",
     "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
     "onsas_tangent_matvec", "onsas_element_forces", "onsas_get_stress_strain",
     "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
@@ -109,6 +110,8 @@ def load():
         "onsas_assemble": ([_vp, _vp], C.c_int),
         "onsas_newton_step": ([_vp, C.POINTER(StepStats)], C.c_int),
         "onsas_get_U": ([_vp, _vp], C.c_int),
+        "onsas_snapshot_U": ([_vp, C.c_int32], C.c_int),
+        "onsas_read_snapshot": ([_vp, C.c_int32, _vp], C.c_int),
         "onsas_set_U": ([_vp, _vp], C.c_int),
         "onsas_get_fext": ([_vp, _vp], C.c_int),
         "onsas_get_fint": ([_vp, _vp], C.c_int),
@@ -264,6 +267,20 @@ class Context:
 
     def get_U(self):
         return self._get(self.L.onsas_get_U, 3 * self.nn)
+
+    def snapshot_U(self, slot):
+        """Enqueue the copy of U into pinned snapshot `slot` (0 or 1); returns at once."""
+        self._check(self.L.onsas_snapshot_U(self.h, slot))
+
+    def read_snapshot(self, slot, out):
+        """Wait for snapshot `slot` and copy it into `out` (3 nn float64).  Uses
+        no stream: safe on another thread while this one drives the context."""
+        if out.dtype != np.float64 or out.size != 3 * self.nn or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float64 array of 3 nn values")
+        rc = self.L.onsas_read_snapshot(self.h, slot, out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"onsas_read_snapshot failed ({rc})")
+        return out
 
     def set_U(self, U):
         u = _f64(U, 3 * self.nn)

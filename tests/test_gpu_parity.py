@@ -662,3 +662,28 @@ def test_newton_with_amg(prec, refresh):
         ex = workloads.homogeneous_displacement(sm.coords, a, b)
         np.testing.assert_allclose(sol.U[k], ex, rtol=0, atol=1e-8 * np.abs(ex).max())
         assert sol.records[k]["dr_rel"] <= 1e-10
+
+
+def test_u_snapshots():
+    """onsas_snapshot_U / onsas_read_snapshot: two slots, read on another
+    thread after later work was enqueued, equal to the U of the moment each
+    snapshot was taken (the per-step copy the Newton driver overlaps)."""
+    from concurrent.futures import ThreadPoolExecutor
+    print("[gpu] U snapshots", flush=True)
+    sm = meshes.cube(3, jitter=0.1, seed=4)
+    c = _ctx("mixed", sm.coords, sm.tets, "nh")
+    rng = np.random.default_rng(12)
+    U0, U1 = rng.normal(size=3 * sm.nn), rng.normal(size=3 * sm.nn)
+    c.set_U(U0)
+    c.snapshot_U(0)
+    c.set_U(U1)
+    c.snapshot_U(1)
+    c.assemble()  # more work on the stream after both snapshots
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        got0 = pool.submit(c.read_snapshot, 0, np.empty(3 * sm.nn)).result()
+        got1 = pool.submit(c.read_snapshot, 1, np.empty(3 * sm.nn)).result()
+    assert np.array_equal(got0, U0) and np.array_equal(got1, U1)
+    assert np.array_equal(c.get_U(), U1)
+    with pytest.raises(_capi.OnsasError):
+        c.snapshot_U(2)
+    c.close()
