@@ -1069,6 +1069,14 @@ int amg_build(Ctx* c) {
   const int64_t gen = A.gen;  // monotonic across rebuilds (graph cache key)
   A = Amg{};
   A.gen = gen + 1;
+  {
+    // degree 1 on the finest level: a damped-Jacobi step (w = 1/theta) before
+    // and after the coarse correction, two tangent products per cycle instead
+    // of four, for ~4 % more PCG iterations at 64^3 (12 % less solve time);
+    // ONSAS_AMG_FINE_DEGREE=2 selects the degree-2 Chebyshev smoother there
+    const char* e = std::getenv("ONSAS_AMG_FINE_DEGREE");
+    A.fine_degree = (e && std::atoi(e) == 2) ? 2 : 1;
+  }
   const int64_t nn = c->nn;
   const bool f64 = c->precision == ONSAS_PRECISION_FP64;
   AmgLevel L0;
@@ -1293,7 +1301,7 @@ int amg_build(Ctx* c) {
   }
   for (void** v : {&A.lev[0].d_x, &A.lev[0].d_r, &A.lev[0].d_d0, &A.lev[0].d_d1, &A.lev[0].d_ev})
     ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));
-  if (f64) ONSAS_TRY(alloc(c, &A.d_Ks, sizeof(float) * 9 * (size_t)c->nsell));
+  if (f64 && A.fine_degree == 2) ONSAS_TRY(alloc(c, &A.d_Ks, sizeof(float) * 9 * (size_t)c->nsell));
   A.ks_stale = true;
   A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
@@ -1501,12 +1509,18 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     AmgLevel& L = A.lev[l];
     const Cheb ch = cheb(L.lam);
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
-    KT* x = (KT*)L.d_x;
     KT* r = (KT*)L.d_r;
-    // x = d0 + d1 (smoother copy at the fine level), then the exact r = b - A x
-    op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
-                nullptr, nullptr, true);
-    op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 0), evs(l, 1));
+    if (l == 0 && A.fine_degree == 1) {
+      // degree 1: x = d0 itself (prolonged into in place), the exact r = b - A d0
+      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st,
+                  evs(l, 0), evs(l, 1));
+    } else {
+      KT* x = (KT*)L.d_x;
+      // x = d0 + d1 (smoother copy at the fine level), then the exact r = b - A x
+      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
+                  nullptr, nullptr, true);
+      op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 0), evs(l, 1));
+    }
     AmgLevel& C = A.lev[l + 1];
     const bool coarsest = l + 2 == nl;
     const KT sc = coarsest ? KT(0) : (KT)cheb(C.lam).inv_theta;
@@ -1527,9 +1541,10 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     const AmgLevel& C = A.lev[l + 1];
     const Cheb ch = cheb(L.lam);
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
-    KT* x = (KT*)L.d_x;
-    KT* r = (KT*)L.d_r;
+    const bool deg1 = l == 0 && A.fine_degree == 1;
     KT* d0 = (KT*)L.d_d0;
+    KT* x = deg1 ? d0 : (KT*)L.d_x;
+    KT* r = (KT*)L.d_r;
     KT* z = l == 0 ? z0 : x;
     if (l == 0)
       hipLaunchKernelGGL((k_sa_prolong_node<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
@@ -1538,9 +1553,15 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
                          L.d_p_ptr, L.d_p_col, (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
-    // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
-    op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
-    op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, nullptr, nullptr, true);
+    if (deg1) {
+      // z = x + D^-1 (b - A x) / theta: the degree-1 step in the product's epilogue
+      op_cheb<KT>(c, l, x, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr, z, st, evs(l, 2),
+                  evs(l, 3));
+    } else {
+      // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
+      op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
+      op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, nullptr, nullptr, true);
+    }
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   return ONSAS_OK;
@@ -1563,7 +1584,7 @@ int amg_prepare(Ctx* c) {
   Amg& A = c->amg;
   if (!A.ks_stale) return ONSAS_OK;
   const int64_t n = 9 * c->nsell;
-  if (c->precision == ONSAS_PRECISION_FP64) {
+  if (c->precision == ONSAS_PRECISION_FP64 && A.fine_degree == 2) {
     hipLaunchKernelGGL(k_to_smoother_f64, dim3(2048), dim3(256), 0, c->stream,
                        (const double*)c->d_Ksell, (float*)A.d_Ks, n);
     ONSAS_HIP_TRY(c, hipGetLastError());

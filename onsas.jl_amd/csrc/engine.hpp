@@ -165,6 +165,7 @@ struct Amg {
   bool ks_stale = true;    // the smoother copy of the fine tangent is older than K
   void* d_Ks = nullptr;    // that copy (FP64 mode): the sliced-ELL values in FP32
   bool step_stale = true;  // ... to an earlier load step
+  int fine_degree = 1;     // Chebyshev degree of the finest level's smoothing (1 or 2)
   std::vector<AmgLevel> lev;
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
   double* d_dense = nullptr;          // nc x nc FP64 work (the inverse in place)
