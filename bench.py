@@ -221,16 +221,19 @@ def main():
     def spmv_roofline(r, kt):
         """The fine-level tangent products: k_pcg_A (PCG: K blocks + col +
         gathered z and p_old, written p_new and q) and, with the multigrid, the
-        cycle's two exact residual products k_sell_cheb<float, float> (K blocks +
-        col, plus per dof 3 and 7 vector values read or written: x, b, r / x, b,
-        block-Jacobi 3, r, d0 -- 5 on average)."""
+        cycle's two fine-level products k_sell_cheb<float, float> (K blocks +
+        col, plus per dof 3 and 6 vector values read or written: x, b, r / x, b,
+        block-Jacobi 3, z -- 4.5 on average; with ONSAS_AMG_FINE_DEGREE=2 the two
+        exact products of the degree-2 cycle: x, b, r / x, b, block-Jacobi 3, r,
+        d0 -- 5)."""
         tm = r["timings"]
         nn, nnzb = r["nn"], r["nnzb"]
         n = 3 * nn
         out = dict(pcg_ms_per_iter=tm["pcg_ms"] / max(tm["pcg_iters"], 1))
+        cyc_vec = 5 if os.environ.get("ONSAS_AMG_FINE_DEGREE") == "2" else 4.5
         for name, ms, cnt, vec in (("k_pcg_A", tm["spmv_ms"], tm["spmv_launches"], 4),
                                    ("k_sell_cheb", tm["cycle_spmv_ms"], tm["cycle_spmv_launches"],
-                                    5)):
+                                    cyc_vec)):
             if cnt <= 0:
                 continue
             b = int(nnzb * (9 * kt + 4) + vec * n * kt)
