@@ -655,8 +655,7 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
 // Block-CSR -> sliced-ELL values: one thread per (round slot, value).
 template <typename KT>
 __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restrict__ A,
-                           KT* __restrict__ A6, typename CoarseSmootherType<KT>::type* __restrict__ A6s,
-                           int64_t nslots) {
+                           typename CoarseSmootherType<KT>::type* __restrict__ A6s, int64_t nslots) {
   using ST = typename CoarseSmootherType<KT>::type;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 36 * nslots) return;
@@ -666,9 +665,7 @@ __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restr
   const int l = (int)(slot & 63);
   const int32_t k = s6_src[slot];
   const KT v = k >= 0 ? A[36 * (int64_t)k + e] : KT(0);
-  const int64_t at = ((9 * q + e / 4) * 64 + l) * 4 + (e & 3);
-  A6[at] = v;
-  A6s[at] = to_st<ST>(v);
+  A6s[((9 * q + e / 4) * 64 + l) * 4 + (e & 3)] = to_st<ST>(v);
 }
 
 // y = s D^-1 v, one thread per (node, component).
@@ -1280,7 +1277,6 @@ int amg_build(Ctx* c) {
       ONSAS_TRY(upload(c, &Lc.d_s6_col, s6_col));
       ONSAS_TRY(upload(c, &Lc.d_s6_src, s6_src));
       ONSAS_TRY(upload(c, &Lc.d_s6_row, s6_row));
-      ONSAS_TRY(alloc(c, &Lc.d_A6, c->kt * 36 * 64 * (size_t)Lc.nrounds6));
       ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(float) : sizeof(uint16_t)) * 36 * 64 *
                                         (size_t)Lc.nrounds6));
     }
@@ -1339,9 +1335,9 @@ void launch_sell6(const AmgLevel& L, const void* A6, const KT* v, const KT* b, K
 }
 
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
-// approx: the product's result only feeds a smoothing update; it may use the
-// smoother copy of the operator (fine level in FP64 mode, sliced-ELL coarse
-// levels)
+// approx: the product's result only feeds a smoothing update; on the fine
+// level in FP64 mode (degree-2 smoothing) it may use the FP32 copy of the
+// tangent.  Coarse levels with a sliced-ELL copy use it for every product.
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
@@ -1369,10 +1365,11 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
   } else {
     const AmgLevel& L = c->amg.lev[l];
     using CT = typename CoarseSmootherType<KT>::type;
-    if (L.nslices6 > 0 && approx)
+    // a level's products all use one operator (its sliced-ELL copy, BF16 /
+    // FP32 in FP64 mode), so its smoothing and residuals stay consistent and
+    // the cycle symmetric positive definite
+    if (L.nslices6 > 0)
       launch_sell6<KT, CT>(L, L.d_A6s, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
-    else if (L.nslices6 > 0)
-      launch_sell6<KT, KT>(L, L.d_A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
     else
       hipLaunchKernelGGL(k_bsr6_cheb<KT>, dim3(nblocks(L.n, 4)), dim3(256), 0, s, L.d_row_ptr,
                          L.d_col, (const KT*)L.d_A, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1,
@@ -1452,7 +1449,7 @@ int amg_numeric_t(Ctx* c) {
                          (KT*)C.d_Dinv, C.n);
       if (C.nslices6 > 0)
         hipLaunchKernelGGL(k_to_sell6<KT>, dim3(nblocks(36 * 64 * C.nrounds6, 256)), dim3(256), 0,
-                           s, C.d_s6_src, (const KT*)C.d_A, (KT*)C.d_A6,
+                           s, C.d_s6_src, (const KT*)C.d_A,
                            (typename CoarseSmootherType<KT>::type*)C.d_A6s, 64 * C.nrounds6);
     }
     ONSAS_HIP_TRY(c, hipGetLastError());
@@ -1689,7 +1686,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_rt_ptr, (void*)L.d_rt_row, (void*)L.d_rt_pos, L.d_PT, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
                     (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w,
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
-                    (void*)L.d_s6_row, L.d_A6, L.d_A6s})
+                    (void*)L.d_s6_row, L.d_A6s})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks})
     if (p) (void)hipFree(p);

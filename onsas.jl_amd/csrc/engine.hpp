@@ -125,13 +125,13 @@ struct AmgLevel {
   // them, one wave each; lane l = R ph + r serves row s6_row[R s + r] (-1:
   // none), slot (64 / R) m + ph of round m.  Round q of the level (s6_ptr[s] <= q < s6_ptr[s+1])
   // holds lane l's block column at s6_col[64 q + l] and its 36 values in 9
-  // chunks of 4 at A6[((9 q + chunk) 64 + l) 4 + j]; s6_src is the source block
-  // in A (-1: padding, zero values).
+  // chunks of 4 at A6s[((9 q + chunk) 64 + l) 4 + j], rounded to BF16 (FP32 in
+  // FP64 mode): every product of the cycle on this level reads that copy.
+  // s6_src is the source block in A (-1: padding, zero values).
   int64_t nslices6 = 0, nrounds6 = 0;
   int rpw6 = 8;                       // rows per slice: 8, 4 or 2
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
-  void* d_A6 = nullptr;
-  void* d_A6s = nullptr;              // its smoothing-only copy (BF16 mixed, FP32 FP64)
+  void* d_A6s = nullptr;              // the sliced-ELL values (BF16 mixed, FP32 FP64)
   double lam = 0;                     // lambda_max(D^-1 A), power iteration
   void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)
   // cycle vectors (KT, bs n): right-hand side, iterate, residual, Chebyshev steps
