@@ -194,7 +194,9 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure,
 
 /* assemble!(s, sa) (src/StructuralAnalyses/StaticAnalyses.jl:99-122): f_int
  * (FP64) and the tangent at the device-resident U.  fint_out: optional n
- * values copied back. */
+ * values copied back (the call then waits for the assembly); without it the
+ * call returns once the work is queued, and every later call sees its result
+ * (they run on the context's one stream). */
 int onsas_assemble(onsas_ctx* ctx, double* fint_out);
 
 /* step!(sa, ::NewtonRaphson, linear_solver)

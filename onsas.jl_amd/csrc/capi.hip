@@ -196,6 +196,18 @@ int assemble_now(Ctx* c) {
   return ONSAS_OK;
 }
 
+// Read the last assembly's event timings (the stream has been synchronized).
+void account_assembly(Ctx* c) {
+  if (!c->asm_timing_pending) return;
+  c->asm_timing_pending = false;
+  c->last_assemble_ms = elapsed(c->ev[0], c->ev[3]);
+  c->tm.assemble_ms += c->last_assemble_ms;
+  c->tm.scatter_ms += elapsed(c->ev[1], c->ev[2]);
+  if (c->gather) c->tm.elem_ms += elapsed(c->ev[1], c->ev[8]);
+  c->tm.assembly_mode = c->gather ? 1 : 0;
+  c->tm.n_assemble++;
+}
+
 }  // namespace
 
 extern "C" {
@@ -418,18 +430,21 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
 int onsas_assemble(onsas_ctx* ctx, double* fint_out) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
+  if (c->asm_timing_pending) {  // the previous assembly's events are about to be reused
+    ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    account_assembly(c);
+  }
   ONSAS_TRY(assemble_now(c));
-  if (fint_out)
+  c->asm_timing_pending = true;
+  c->assembled = true;
+  if (fint_out) {
     ONSAS_HIP_TRY(c, hipMemcpyAsync(fint_out, c->d_fint, 3 * c->nn * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream));
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  c->last_assemble_ms = elapsed(c->ev[0], c->ev[3]);
-  c->tm.assemble_ms += c->last_assemble_ms;
-  c->tm.scatter_ms += elapsed(c->ev[1], c->ev[2]);
-  if (c->gather) c->tm.elem_ms += elapsed(c->ev[1], c->ev[8]);
-  c->tm.assembly_mode = c->gather ? 1 : 0;
-  c->tm.n_assemble++;
-  c->assembled = true;
+    ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    account_assembly(c);
+  }
+  // otherwise the work stays queued: the next call that synchronizes (the Newton
+  // step) reads the timings, and the host does not wait on the GPU in between
   return ONSAS_OK;
 }
 
@@ -453,6 +468,7 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
   ONSAS_HIP_TRY(c, hipMemcpyAsync(&c->h_sc->nwt, &c->d_sc->nwt, sizeof(onsas::NewtonStats),
                                   hipMemcpyDeviceToHost, s));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
+  account_assembly(c);
   const onsas::NewtonStats& nw = c->h_sc->nwt;
   out->r_norm = std::sqrt(nw.r2);
   out->fext_norm = std::sqrt(nw.fext2);
@@ -818,6 +834,10 @@ int onsas_locate_points(onsas_ctx* ctx, int32_t group, int64_t npts, const doubl
 int onsas_get_timings(onsas_ctx* ctx, onsas_timings* t) {
   Ctx* c = C(ctx);
   if (!c || !t) return ONSAS_ERR_ARG;
+  if (c->asm_timing_pending) {
+    ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    account_assembly(c);
+  }
   *t = c->tm;
   return ONSAS_OK;
 }

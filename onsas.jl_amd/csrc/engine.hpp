@@ -306,6 +306,7 @@ struct Ctx {
   hipEvent_t ev_spmv[2 * 32] = {};  // brackets of the sampled SpMV launches
   onsas_timings tm{};
   double last_assemble_ms = 0;
+  bool asm_timing_pending = false;  // an assembly's events not yet read (read after a sync)
   bool assembled = false;
 };
 
