@@ -741,6 +741,28 @@ __global__ void k_residual_kt(const double* __restrict__ b, const KT* __restrict
 
 // One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
 // optional events around pcg_A.
+// Convergence check right after the update (multigrid PCG): ||r_{k+1}||^2 from
+// the update's partials, summed exactly as the next k_pcg_A sums them (same
+// helper, same width), and the same stopping rule, so the iteration count is
+// unchanged; a converged solve then skips the preconditioner application of
+// its last residual instead of computing it and stopping one kernel later.
+template <int T>
+__global__ __launch_bounds__(T) void k_pcg_check(const double* __restrict__ prr,
+                                                 PcgState* __restrict__ st, int np) {
+  __shared__ double sh[3 * T / 64];
+  if (st->done | st->breakdown) return;
+  double unused0, rr, unused2;
+  sum_parts3<T>(prr, prr, nullptr, np, sh, unused0, rr, unused2);
+  const int k = st->k;  // iterations completed
+  const bool conv = rr <= st->tol2;
+  if (threadIdx.x == 0 && (conv || k >= st->maxiter)) {
+    st->done = 1;
+    st->iters = k;
+    st->converged = conv ? 1 : 0;
+    st->rr_last = rr;
+  }
+}
+
 template <typename KT>
 void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEvent_t* evc = nullptr) {
   hipStream_t s = c->stream;
@@ -766,6 +788,8 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
                        (const KT*)c->d_Dinv, (KT)s0, d0, c->np, c->chunk, c->nn);
+    hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
+                       part(c, kPartRR), st, c->np);
     (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc);
     hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
                        (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
