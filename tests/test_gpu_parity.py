@@ -664,6 +664,41 @@ def test_newton_with_amg(prec, refresh):
         assert sol.records[k]["dr_rel"] <= 1e-10
 
 
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_amg_pcg_maxiter_exact(prec):
+    """The multigrid PCG stops exactly where the reference's cg! would: with
+    maxiter = N (N the iterations it needs) it converges in N iterations to the
+    same x, and with N - 1 it reports no convergence.  The check runs right
+    after each update and again at the next product, and N falls at different
+    places of the 2 + 8 + 8 ... launch chunks for the three tolerances."""
+    print(f"[gpu] AMG PCG maxiter {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=10, jitter=0.05, seed=2)
+    fixed = np.asarray(s.fixed_dofs)
+    a = workloads.nh_uniaxial_alpha(-p * 0.5, 8.3e5, 3.8e5)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    U = workloads.homogeneous_displacement(sm.coords, a, b)
+    c = _ctx(prec, sm.coords, sm.tets, "nh", fixed=fixed)
+    c.set_U(U)
+    c.assemble()
+    rhs = np.random.default_rng(7).normal(size=3 * sm.nn)
+    seen = set()
+    for rtol in (1e-2, 1e-4, 1e-6 if prec == "mixed" else 1e-9):
+        c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=500, precond=_capi.PRECOND_AMG)
+        x, n, _ = c.linear_solve(rhs)
+        seen.add(n)
+        c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=n, precond=_capi.PRECOND_AMG)
+        x2, n2, _ = c.linear_solve(rhs)
+        assert n2 == n and np.array_equal(x, x2)
+        if n > 1:
+            c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=n - 1,
+                                precond=_capi.PRECOND_AMG)
+            with pytest.raises(_capi.OnsasError):
+                c.linear_solve(rhs)
+    print(f"  iteration counts {sorted(seen)}", flush=True)
+    assert len(seen) >= 2
+    c.close()
+
 def test_u_snapshots():
     """onsas_snapshot_U / onsas_read_snapshot: two slots, read on another
     thread after later work was enqueued, equal to the U of the moment each
