@@ -955,15 +955,16 @@ int upload_kt(Ctx* c, void** d, const std::vector<double>& h) {
   return upload(c, (KT**)d, v);
 }
 
-// Rows per sliced-ELL slice of a coarse level: fewer rows per wave split a row
-// over more lanes (fewer dependent index -> gather rounds, more waves in flight)
-// at the price of more padding; ONSAS_SELL6_RPW (8, 4 or 2) overrides.
-int sell6_rows_per_wave(int64_t rows) {
+// Rows per sliced-ELL slice of a coarse level.  Fewer rows per wave split a
+// row over more lanes (fewer dependent index -> gather rounds, more waves in
+// flight) at the price of more padding; on the 64^3 cube 8, 4 and 2 measured
+// the same (these launches are bound by latency, not bytes), so 8 (the least
+// padding).  ONSAS_SELL6_RPW (8, 4 or 2) overrides.
+int sell6_rows_per_wave() {
   if (const char* e = std::getenv("ONSAS_SELL6_RPW")) {
     const int r = std::atoi(e);
     if (r == 2 || r == 4 || r == 8) return r;
   }
-  (void)rows;
   return 8;
 }
 
@@ -1218,7 +1219,7 @@ int amg_build(Ctx* c) {
       for (int64_t w0 = 0; w0 < na; w0 += kSell6Window)
         std::stable_sort(order.begin() + w0, order.begin() + std::min<int64_t>(w0 + kSell6Window, na),
                          [&](int32_t x, int32_t y) { return len(x) > len(y); });
-      const int rpw = sell6_rows_per_wave(na);
+      const int rpw = sell6_rows_per_wave();
       const int lpr = 64 / rpw;
       Lc.rpw6 = rpw;
       for (int64_t r0 = 0; r0 < na; r0 += rpw) {
