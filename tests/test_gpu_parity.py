@@ -699,6 +699,39 @@ def test_amg_pcg_maxiter_exact(prec):
     assert len(seen) >= 2
     c.close()
 
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_amg_fine_degree_two(oracle, prec, monkeypatch):
+    """The degree-2 Chebyshev fine smoother (ONSAS_AMG_FINE_DEGREE=2, read when
+    the hierarchy is built) solves the same system as the default degree-1
+    step, to the same tolerance, in no more iterations."""
+    import scipy.sparse.linalg as spla
+    print(f"[gpu] AMG fine degree 2 {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=8, jitter=0.05, seed=2)
+    fixed = s.fixed_dofs
+    a = workloads.nh_uniaxial_alpha(-p * 0.5, 8.3e5, 3.8e5)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    U = workloads.homogeneous_displacement(sm.coords, a, b)
+    rtol = 1e-10 if prec == "fp64" else 1e-6
+    rhs = np.random.default_rng(3).normal(size=3 * sm.nn)
+    Kff = _oracle_model(oracle, sm.coords, sm.tets, "nh", fixed).assemble(
+        U, want_full=False, want_ff=True)["Kff"]
+    free = np.setdiff1d(np.arange(3 * sm.nn), fixed)
+    xo = spla.spsolve(Kff.tocsc(), rhs[free])
+    its = {}
+    for deg in ("1", "2"):
+        monkeypatch.setenv("ONSAS_AMG_FINE_DEGREE", deg)
+        c = _ctx(prec, sm.coords, sm.tets, "nh", fixed=fixed)
+        c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=500, precond=_capi.PRECOND_AMG)
+        c.set_U(U)
+        c.assemble()
+        x, its[deg], _ = c.linear_solve(rhs)
+        err = np.abs(x[free] - xo).max() / np.abs(xo).max()
+        assert err < (1e-8 if prec == "fp64" else 2e-4), (deg, err)
+        c.close()
+    print(f"  iterations degree 1 {its['1']}, degree 2 {its['2']}", flush=True)
+    assert its["2"] <= its["1"]
+
 def test_u_snapshots():
     """onsas_snapshot_U / onsas_read_snapshot: two slots, read on another
     thread after later work was enqueued, equal to the U of the moment each
