@@ -92,14 +92,14 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // Solve K x = b_dev with the configured linear solver; x left in the PCG x buffer (KT).
 int linear_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
-                 int* converged, int* solver) {
+                 int* converged, int* solver, const void* x0 = nullptr) {
   int it = 0, conv = 0, brk = 0;
   double rr = 1.0;
   *solver = 0;
   *iters = 0;
   const int kind = c->lin.kind;
   if (kind == ONSAS_LINSOLVE_PCG || kind == ONSAS_LINSOLVE_AUTO) {
-    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, forcing, &it, &rr, &conv, &brk));
+    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, forcing, &it, &rr, &conv, &brk, x0));
     *solver = 1;
     *iters = it;
     c->tm.pcg_iters += it;
@@ -274,6 +274,9 @@ void onsas_destroy(onsas_ctx* ctx) {
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (double* h : c->h_snap)
     if (h) (void)hipHostFree(h);
+  for (auto& bank : c->d_warm)
+    for (void* p : bank)
+      if (p) (void)hipFree(p);
   for (auto& e : c->ev_snap)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->ev)
@@ -411,6 +414,10 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
   if (npressure != (int32_t)c->pressure_faces.size() || (npressure > 0 && !pressures))
     return set_err(c, ONSAS_ERR_ARG, "onsas_set_loads: one pressure value per pressure group expected");
   const size_t n = 3 * (size_t)c->nn;
+  // a new load step: last step's updates become the warm starts
+  c->warm_cur ^= 1;
+  for (bool& ok : c->warm_ok[c->warm_cur]) ok = false;
+  c->newton_it = 0;
   if (nodal) {
     std::memcpy(c->h_stage, nodal, n * sizeof(double));
     ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_nodal, c->h_stage, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -460,7 +467,18 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
   int iters = 0, conv = 0, solver = 0;
   double relres = 0;
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[5], s));
-  ONSAS_TRY(linear_solve(c, c->d_res, true, &iters, &relres, &conv, &solver));
+  // warm start from the previous load step's update of this Newton iteration
+  const int j = c->newton_it++;
+  const int prev = c->warm_cur ^ 1;
+  const bool slot = c->warm_start && j < Ctx::kWarmSlots;
+  const void* x0 = slot && c->warm_ok[prev][j] ? c->d_warm[prev][j] : nullptr;
+  ONSAS_TRY(linear_solve(c, c->d_res, true, &iters, &relres, &conv, &solver, x0));
+  if (slot) {  // this step's update, for the next step (PCG solutions only)
+    c->warm_ok[c->warm_cur][j] = solver == 1;
+    if (solver == 1)
+      ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_warm[c->warm_cur][j], c->d_px, c->kt * 3 * c->nn,
+                                      hipMemcpyDeviceToDevice, s));
+  }
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[6], s));
   ONSAS_TRY(onsas::launch_update(c));
   ONSAS_TRY(onsas::launch_finalize_newton(c));
@@ -526,6 +544,9 @@ int onsas_set_U(onsas_ctx* ctx, const double* U) {
   if (!U) return set_err(c, ONSAS_ERR_ARG, "NULL input");
   ONSAS_TRY(upload_state(c, c->d_U, U));
   c->assembled = false;
+  for (auto& bank : c->warm_ok)  // another state: no warm starts from the old one
+    for (bool& ok : bank) ok = false;
+  c->newton_it = 0;
   return ONSAS_OK;
 }
 

@@ -291,6 +291,16 @@ struct Ctx {
     int64_t gen = -1;
   } pcg_graph[2];
   int64_t pcg_solves = 0;  // PCG solves so far (every kSampleEvery-th is timed)
+  // Warm starts of the Newton solves: the PCG solution of Newton iteration j
+  // (j < kWarmSlots) of the current load step goes to warm[cur][j]; iteration j
+  // of the next step starts from it (scaled on the device).  set_loads flips
+  // the banks, set_U empties both.
+  static constexpr int kWarmSlots = 4;
+  void* d_warm[2][kWarmSlots] = {};
+  bool warm_ok[2][kWarmSlots] = {};
+  int warm_cur = 0;
+  int newton_it = 0;        // Newton iterations so far in this load step
+  bool warm_start = true;   // ONSAS_WARM_START=0 turns it off (read at finalize)
 
   // aggregation multigrid preconditioner (built on its first use)
   Amg amg;
@@ -352,8 +362,10 @@ int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2
+// x0: optional warm start (n KT values, zero on fixed dofs), scaled on the
+// device by the factor minimising ||b - a K x0||
 int launch_pcg_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
-                     int* converged, int* breakdown);
+                     int* converged, int* breakdown, const void* x0 = nullptr);
 double linear_rtol(const Ctx* c);                               // configured / default rtol
 int reset_forcing(Ctx* c);                                      // new load step
 int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged);

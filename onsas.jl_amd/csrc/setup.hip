@@ -412,6 +412,13 @@ int finalize_model(Ctx* c) {
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_stage, sizeof(double) * n, hipHostMallocDefault));
   for (double*& h : c->h_snap)
     ONSAS_HIP_TRY(c, hipHostMalloc((void**)&h, sizeof(double) * n, hipHostMallocDefault));
+  {
+    const char* w = std::getenv("ONSAS_WARM_START");
+    c->warm_start = !(w && w[0] == '0');
+  }
+  if (c->warm_start)
+    for (auto& bank : c->d_warm)
+      for (void*& p : bank) ONSAS_TRY(dev_alloc(c, &p, c->kt * n));
 
   // reduction grid: one row per thread when it fits, whole 64-row slices per
   // workgroup, a multiple of 8 workgroups (XCD mapping), at most kMaxPartials
