@@ -732,6 +732,30 @@ def test_amg_fine_degree_two(oracle, prec, monkeypatch):
     print(f"  iterations degree 1 {its['1']}, degree 2 {its['2']}", flush=True)
     assert its["2"] <= its["1"]
 
+
+@pytest.mark.parametrize("pc", ["block-jacobi", "amg"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_warm_start_matches_cold(prec, pc, monkeypatch):
+    """Warm-started Newton solves (each PCG starts from the previous load
+    step's update of the same Newton iteration, scaled to minimise the initial
+    residual) take the same Newton iterations to the same states as zero
+    starts (ONSAS_WARM_START=0), with fewer PCG iterations."""
+    print(f"[gpu] warm vs cold start {prec} {pc}", flush=True)
+    s, sm, p = workloads.cube_compression(n=6, jitter=0.05, seed=2)
+    runs = {}
+    for w in ("0", "1"):
+        monkeypatch.setenv("ONSAS_WARM_START", w)
+        runs[w] = _newton(s, 6, prec, tol_f=1e-9, preconditioner=pc, forcing=True)
+    cold, warm = runs["0"], runs["1"]
+    assert warm.iterations() == cold.iterations()
+    for k in range(6):
+        np.testing.assert_allclose(warm.U[k], cold.U[k], rtol=0,
+                                   atol=1e-8 * np.abs(cold.U[k]).max())
+        assert warm.records[k]["dr_rel"] <= 1e-9
+    print(f"  PCG iterations cold {cold.linear_iterations()} warm {warm.linear_iterations()}",
+          flush=True)
+    assert warm.linear_iterations() < cold.linear_iterations()
+
 def test_u_snapshots():
     """onsas_snapshot_U / onsas_read_snapshot: two slots, read on another
     thread after later work was enqueued, equal to the U of the moment each
