@@ -962,6 +962,10 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int* 
                      c->lin.newton_tol, c->lin.eta_max > 0 ? c->lin.eta_max : 0.1,
                      c->lin.eta_first > 0 ? c->lin.eta_first : 1e-3);
   if (amg) {
+    // a (warm) start that already meets the tolerance stops here, before the
+    // preconditioner application of r_0 (same rule as the first k_pcg_A)
+    hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
+                       part(c, kPartRR), &c->d_sc->pcg, np);
     ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, false));
     hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
                        (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk, c->nn);
