@@ -92,14 +92,14 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // Solve K x = b_dev with the configured linear solver; x left in the PCG x buffer (KT).
 int linear_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
-                 int* converged, int* solver, const void* x0 = nullptr) {
+                 int* converged, int* solver, const void* x0 = nullptr, int expect = -1) {
   int it = 0, conv = 0, brk = 0;
   double rr = 1.0;
   *solver = 0;
   *iters = 0;
   const int kind = c->lin.kind;
   if (kind == ONSAS_LINSOLVE_PCG || kind == ONSAS_LINSOLVE_AUTO) {
-    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, forcing, &it, &rr, &conv, &brk, x0));
+    ONSAS_TRY(onsas::launch_pcg_solve(c, b_dev, forcing, &it, &rr, &conv, &brk, x0, expect));
     *solver = 1;
     *iters = it;
     c->tm.pcg_iters += it;
@@ -417,6 +417,7 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
   // a new load step: last step's updates become the warm starts
   c->warm_cur ^= 1;
   for (bool& ok : c->warm_ok[c->warm_cur]) ok = false;
+  for (int& n : c->pcg_pred[c->warm_cur]) n = -1;
   c->newton_it = 0;
   if (nodal) {
     std::memcpy(c->h_stage, nodal, n * sizeof(double));
@@ -472,7 +473,10 @@ int onsas_newton_step(onsas_ctx* ctx, onsas_step_stats* out) {
   const int prev = c->warm_cur ^ 1;
   const bool slot = c->warm_start && j < Ctx::kWarmSlots;
   const void* x0 = slot && c->warm_ok[prev][j] ? c->d_warm[prev][j] : nullptr;
-  ONSAS_TRY(linear_solve(c, c->d_res, true, &iters, &relres, &conv, &solver, x0));
+  const bool pslot = c->pcg_predict && j < Ctx::kPredSlots;
+  ONSAS_TRY(linear_solve(c, c->d_res, true, &iters, &relres, &conv, &solver, x0,
+                         pslot ? c->pcg_pred[prev][j] : -1));
+  if (pslot) c->pcg_pred[c->warm_cur][j] = solver == 1 ? iters : -1;
   if (slot) {  // this step's update, for the next step (PCG solutions only)
     c->warm_ok[c->warm_cur][j] = solver == 1;
     if (solver == 1)
@@ -546,6 +550,8 @@ int onsas_set_U(onsas_ctx* ctx, const double* U) {
   c->assembled = false;
   for (auto& bank : c->warm_ok)  // another state: no warm starts from the old one
     for (bool& ok : bank) ok = false;
+  for (auto& bank : c->pcg_pred)
+    for (int& n : bank) n = -1;
   c->newton_it = 0;
   return ONSAS_OK;
 }

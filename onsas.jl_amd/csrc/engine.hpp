@@ -282,14 +282,15 @@ struct Ctx {
   // linear solver settings
   onsas_linear_settings lin{};
 
-  // PCG graphs: [0] the opening iterations of a solve, [1] the repeated chunk;
-  // each keyed by its length, preconditioner and multigrid generation
+  // PCG graphs: [0] 2 iterations (a solve's opening), [1] the repeated chunk,
+  // [2] 4 iterations; each keyed by its length, preconditioner and multigrid
+  // generation
   struct PcgGraph {
     hipGraphExec_t exec = nullptr;
     int iters = 0;
     int precond = -1;
     int64_t gen = -1;
-  } pcg_graph[2];
+  } pcg_graph[3];
   int64_t pcg_solves = 0;  // PCG solves so far (every kSampleEvery-th is timed)
   // Warm starts of the Newton solves: the PCG solution of Newton iteration j
   // (j < kWarmSlots) of the current load step goes to warm[cur][j]; iteration j
@@ -301,6 +302,14 @@ struct Ctx {
   int warm_cur = 0;
   int newton_it = 0;        // Newton iterations so far in this load step
   bool warm_start = true;   // ONSAS_WARM_START=0 turns it off (read at finalize)
+  // PCG iterations of Newton iteration j (j < kPredSlots) of the current load
+  // step (bank warm_cur) and of the previous one (-1: unknown).  A solve
+  // launches the previous step's count before its first host synchronization;
+  // banks flip and clear with the warm-start banks.  ONSAS_PCG_PREDICT=0 turns
+  // it off (read at finalize).
+  static constexpr int kPredSlots = 8;
+  int pcg_pred[2][kPredSlots] = {{-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1}};
+  bool pcg_predict = true;
 
   // aggregation multigrid preconditioner (built on its first use)
   Amg amg;
@@ -365,7 +374,7 @@ int launch_residual(Ctx* c);                                    // d_res, partia
 // x0: optional warm start (n KT values, zero on fixed dofs), scaled on the
 // device by the factor minimising ||b - a K x0||
 int launch_pcg_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
-                     int* converged, int* breakdown, const void* x0 = nullptr);
+                     int* converged, int* breakdown, const void* x0 = nullptr, int expect = -1);
 double linear_rtol(const Ctx* c);                               // configured / default rtol
 int reset_forcing(Ctx* c);                                      // new load step
 int launch_gmres_solve(Ctx* c, const double* b_dev, int* iters, double* relres, int* converged);

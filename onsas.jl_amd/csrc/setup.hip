@@ -415,6 +415,8 @@ int finalize_model(Ctx* c) {
   {
     const char* w = std::getenv("ONSAS_WARM_START");
     c->warm_start = !(w && w[0] == '0');
+    const char* q = std::getenv("ONSAS_PCG_PREDICT");
+    c->pcg_predict = !(q && q[0] == '0');
   }
   if (c->warm_start)
     for (auto& bank : c->d_warm)

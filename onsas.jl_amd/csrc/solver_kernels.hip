@@ -912,7 +912,7 @@ int build_pcg_graph(Ctx* c, int slot, int iters) {
 }
 
 template <typename KT>
-int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int* iters,
+int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int expect, int* iters,
                 double* relres, int* converged, int* breakdown) {
   hipStream_t s = c->stream;
   const int np = c->np;
@@ -979,7 +979,11 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int* 
   }();
   // The opening kFirstIters iterations, then a convergence check, then chunks
   // of chunk_iters iterations with a check after each (both even, so every
-  // replay starts at parity 0 as captured).  Every kSampleEvery-th solve (every
+  // replay starts at parity 0 as captured).  With an expected count `expect`,
+  // the iterations up to it follow the opening before the first check, and a
+  // multigrid solve then goes on 2 iterations per check (its iterations are
+  // dear, a check is not).  Iterations past convergence exit at once on the
+  // device, so the launch shape never changes the result.  Every kSampleEvery-th solve (every
   // solve without graphs) launches the opening iterations directly, the last of
   // them with its tangent products bracketed by dispatch-stamped events on this
   // stream: the sampled durations (the launches right after the solve's host
@@ -991,7 +995,26 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int* 
   if (use_graph) {
     ONSAS_TRY(build_pcg_graph<KT>(c, 0, kFirstIters));
     ONSAS_TRY(build_pcg_graph<KT>(c, 1, chunk_iters));
+    ONSAS_TRY(build_pcg_graph<KT>(c, 2, 4));
   }
+  // n (even) more iterations: chunk_iters at a time, then 4, then 2
+  auto run_iters = [&](int n) -> int {
+    while (n > 0) {
+      const int slot = n >= chunk_iters ? 1 : (n >= 4 ? 2 : 0);
+      const int len = slot == 1 ? chunk_iters : (slot == 2 ? 4 : kFirstIters);
+      if (use_graph) {
+        ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph[slot].exec, s));
+      } else {
+        for (int it = 0; it < len; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
+        ONSAS_HIP_TRY(c, hipGetLastError());
+      }
+      n -= len;
+    }
+    return ONSAS_OK;
+  };
+  int ahead = expect > kFirstIters ? expect - kFirstIters : 0;
+  ahead += ahead & 1;
+  const int more = expect >= 0 && amg ? kFirstIters : chunk_iters;
   const int nsample = kFirstIters;
   hipEvent_t* evc = c->ev_spmv + 32;  // the V-cycle's two exact fine-level products (AMG)
   static_assert(32 + 8 <= sizeof(c->ev_spmv) / sizeof(hipEvent_t), "events");
@@ -1007,14 +1030,7 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int* 
   ONSAS_HIP_TRY(c, hipGetLastError());
   bool first = true;
   for (;;) {
-    if (!first) {
-      if (use_graph) {
-        ONSAS_HIP_TRY(c, hipGraphLaunch(c->pcg_graph[1].exec, s));
-      } else {
-        for (int it = 0; it < chunk_iters; ++it) launch_pcg_iteration<KT>(c, it & 1, nullptr, nullptr);
-        ONSAS_HIP_TRY(c, hipGetLastError());
-      }
-    }
+    ONSAS_TRY(run_iters(first ? ahead : more));
     ONSAS_HIP_TRY(c, hipMemcpyAsync(&c->h_sc->pcg, &c->d_sc->pcg, sizeof(PcgState),
                                     hipMemcpyDeviceToHost, s));
     ONSAS_HIP_TRY(c, hipStreamSynchronize(s));
@@ -1114,10 +1130,10 @@ int launch_residual(Ctx* c) {
 }
 
 int launch_pcg_solve(Ctx* c, const double* b_dev, bool forcing, int* iters, double* relres,
-                     int* converged, int* breakdown, const void* x0) {
+                     int* converged, int* breakdown, const void* x0, int expect) {
   return c->precision == ONSAS_PRECISION_FP64
-             ? pcg_solve_t<double>(c, b_dev, forcing, x0, iters, relres, converged, breakdown)
-             : pcg_solve_t<float>(c, b_dev, forcing, x0, iters, relres, converged, breakdown);
+             ? pcg_solve_t<double>(c, b_dev, forcing, x0, expect, iters, relres, converged, breakdown)
+             : pcg_solve_t<float>(c, b_dev, forcing, x0, expect, iters, relres, converged, breakdown);
 }
 
 double linear_rtol(const Ctx* c) {

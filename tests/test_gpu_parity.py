@@ -756,6 +756,28 @@ def test_warm_start_matches_cold(prec, pc, monkeypatch):
           flush=True)
     assert warm.linear_iterations() < cold.linear_iterations()
 
+
+@pytest.mark.parametrize("pc,chunk", [("block-jacobi", None), ("amg", None), ("amg", 6)])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pcg_predicted_launches_bit_identical(prec, pc, chunk, monkeypatch):
+    """Launching the previous load step's PCG iteration count before the first
+    host check (graphs of chunk, 4 and 2 iterations) changes only how many
+    iterations are queued past convergence, which exit on the device: every
+    state and every PCG count equals that of ONSAS_PCG_PREDICT=0 bit for bit."""
+    print(f"[gpu] predicted PCG launches {prec} {pc} chunk {chunk}", flush=True)
+    s, sm, p = workloads.cube_compression(n=6, jitter=0.05, seed=2)
+    runs = {}
+    for w in ("0", "1"):
+        monkeypatch.setenv("ONSAS_PCG_PREDICT", w)
+        runs[w] = _newton(s, 6, prec, tol_f=1e-9, preconditioner=pc, forcing=True,
+                          check_every=chunk)
+    off, on = runs["0"], runs["1"]
+    assert [d["lin_iters"] for d in on.newton_log] == [d["lin_iters"] for d in off.newton_log]
+    for k in range(6):
+        assert np.array_equal(on.U[k], off.U[k])
+        assert on.records[k]["dr_rel"] == off.records[k]["dr_rel"]
+
+
 def test_u_snapshots():
     """onsas_snapshot_U / onsas_read_snapshot: two slots, read on another
     thread after later work was enqueued, equal to the U of the moment each
