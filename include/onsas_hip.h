@@ -249,6 +249,14 @@ int onsas_amg_get_operator(onsas_ctx* ctx, int32_t level, int32_t* row_ptr, int3
  * FP64 copy of the stored FP16 (mixed) / FP32 (FP64 mode) values). */
 int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int32_t* p_col,
                               double* vals);
+/* Measurement (no counterpart in the reference): `launches` (2..10000)
+ * back-to-back launches of the multigrid cycle's two fine-level tangent
+ * products, alternating, on the last solve's vectors, between two events on
+ * the context's stream; *ms_per_launch = elapsed / launches.  Spreading the
+ * events' own cost over the batch gives the kernel's duration as a kernel
+ * trace reports it.  Needs the hierarchy (onsas_amg_levels > 0); scribbles
+ * on the cycle's work vectors only. */
+int onsas_time_cycle_products(onsas_ctx* ctx, int32_t launches, double* ms_per_launch);
 /* y = K x with the assembled tangent (n values each). */
 int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y);
 

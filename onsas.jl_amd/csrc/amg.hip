@@ -1599,6 +1599,53 @@ double amg_fine_smoother(Ctx* c, void** d0) {
 // ---- hierarchy inspection (tests, diagnostics) -------------------------------------
 
 // the hierarchy has values once a solve has run its numeric setup (lam_set)
+namespace {
+// the cycle's two fine-level products, alternating, on the last solve's
+// vectors (no stop flag), between two events
+template <typename KT>
+int amg_time_fine_products_t(Ctx* c, int launches, double* ms) {
+  Amg& A = c->amg;
+  AmgLevel& L = A.lev[0];
+  const Cheb ch = cheb(L.lam);
+  hipStream_t s = c->stream;
+  const KT* b = (const KT*)c->d_pr;
+  hipEvent_t e0 = c->ev_spmv[2], e1 = c->ev_spmv[3];
+  ONSAS_HIP_TRY(c, hipEventRecord(e0, s));
+  for (int i = 0; i < launches; ++i) {
+    if (A.fine_degree == 1) {  // r = b - A d0;  z = d0 + D^-1 (b - A d0) / theta
+      if (i & 1)
+        op_cheb<KT>(c, 0, (const KT*)L.d_d0, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr,
+                    (KT*)L.d_x, nullptr);
+      else
+        op_cheb<KT>(c, 0, (const KT*)L.d_d0, b, (KT*)L.d_r, 0.0, 0.0, nullptr, nullptr, nullptr,
+                    nullptr);
+    } else {  // r = b - A x;  r and d = D^-1 r / theta
+      if (i & 1)
+        op_cheb<KT>(c, 0, (const KT*)L.d_x, b, (KT*)L.d_r, 0.0, ch.inv_theta, (KT*)L.d_d1,
+                    nullptr, nullptr, nullptr);
+      else
+        op_cheb<KT>(c, 0, (const KT*)L.d_x, b, (KT*)L.d_r, 0.0, 0.0, nullptr, nullptr, nullptr,
+                    nullptr);
+    }
+  }
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  ONSAS_HIP_TRY(c, hipEventRecord(e1, s));
+  ONSAS_HIP_TRY(c, hipEventSynchronize(e1));
+  float t = 0;
+  ONSAS_HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+  *ms = (double)t / launches;
+  return ONSAS_OK;
+}
+}  // namespace
+
+int amg_time_fine_products(Ctx* c, int launches, double* ms) {
+  if (!c->amg.built || !c->amg.lam_set)
+    return set_err(c, ONSAS_ERR_STATE, "no multigrid hierarchy: run a PCG solve with ONSAS_PRECOND_AMG first");
+  if (launches < 2 || launches > 10000) return set_err(c, ONSAS_ERR_ARG, "launches must be in [2, 10000]");
+  return c->precision == ONSAS_PRECISION_FP64 ? amg_time_fine_products_t<double>(c, launches, ms)
+                                              : amg_time_fine_products_t<float>(c, launches, ms);
+}
+
 int amg_nlevels(Ctx* c) { return c->amg.built && c->amg.lam_set ? (int)c->amg.lev.size() : 0; }
 
 int amg_export_info(Ctx* c, int l, int64_t* n, int32_t* bs, int64_t* nnzb, int64_t* nnz_p,

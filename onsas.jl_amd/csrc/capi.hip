@@ -605,6 +605,13 @@ int onsas_get_tangent(onsas_ctx* ctx, double* vals) {
   return ONSAS_OK;
 }
 
+int onsas_time_cycle_products(onsas_ctx* ctx, int32_t launches, double* ms_per_launch) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!ms_per_launch) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  return onsas::amg_time_fine_products(c, launches, ms_per_launch);
+}
+
 int onsas_amg_levels(onsas_ctx* ctx, int32_t* nlevels) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));

@@ -368,6 +368,7 @@ int amg_export_info(Ctx* c, int l, int64_t* n, int32_t* bs, int64_t* nnzb, int64
                     double* lam);
 int amg_export_operator(Ctx* c, int l, int32_t* row_ptr, int32_t* col, double* vals);
 int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double* vals);
+int amg_time_fine_products(Ctx* c, int launches, double* ms);  // batch timing
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2

@@ -38,6 +38,7 @@ EXPORTED = (
     "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
     "onsas_get_timings", "onsas_reset_timings", "onsas_inject_error", "onsas_amg_levels",
     "onsas_amg_level_info", "onsas_amg_get_operator", "onsas_amg_get_prolongator",
+    "onsas_time_cycle_products",
 )
 
 
@@ -132,6 +133,7 @@ def load():
         "onsas_amg_level_info": ([_vp, C.c_int32, _i64p, _i32p, _i64p, _i64p, _dp], C.c_int),
         "onsas_amg_get_operator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
         "onsas_amg_get_prolongator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
+        "onsas_time_cycle_products": ([_vp, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -341,6 +343,13 @@ class Context:
         self._check(self.L.onsas_amg_get_prolongator(self.h, level, rp.ctypes.data, col.ctypes.data,
                                                      vals.ctypes.data))
         return sp.bsr_matrix((vals.reshape(-1, bs, 6), col, rp), shape=(bs * info["n"], 6 * nc))
+
+    def time_cycle_products(self, launches=50):
+        """ms per launch of the multigrid cycle's fine-level tangent products,
+        `launches` back-to-back launches between two events on the stream."""
+        ms = C.c_double(0)
+        self._check(self.L.onsas_time_cycle_products(self.h, int(launches), C.byref(ms)))
+        return ms.value
 
     def tangent_bsr(self):
         """The assembled tangent as a scipy BSR matrix (3x3 blocks, FP64 copy)."""

@@ -566,6 +566,8 @@ def test_amg_hierarchy_galerkin(prec):
     c.finalize()
     c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=1e-6, maxiter=500, precond=_capi.PRECOND_AMG)
     assert c.amg_levels() == 0
+    with pytest.raises(_capi.OnsasError):  # no hierarchy to time yet
+        c.time_cycle_products(10)
     c.set_U(U)
     c.set_loads(None, [0.05 * p])
     c.assemble()
@@ -575,6 +577,14 @@ def test_amg_hierarchy_galerkin(prec):
     print(f"  iterations {it}", flush=True)
     assert it <= 19
     assert c.amg_levels() == 3
+    # the timing batch uses only the cycle's work vectors: the next solve is unchanged
+    ms = c.time_cycle_products(20)
+    print(f"  fine-level cycle product {ms * 1e3:.1f} us", flush=True)
+    assert 0 < ms < 100
+    with pytest.raises(_capi.OnsasError):
+        c.time_cycle_products(1)
+    x2, it2, _ = c.linear_solve(f)
+    assert it2 == it and np.array_equal(x2, x)
     info = [c.amg_level_info(l) for l in range(3)]
     assert [i["bs"] for i in info] == [3, 6, 6]
     assert info[0]["n"] == sm.nn and info[2]["nnz_p"] == 0
