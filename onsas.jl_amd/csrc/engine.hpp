@@ -308,6 +308,7 @@ struct Ctx {
   // banks flip and clear with the warm-start banks.  ONSAS_PCG_PREDICT=0 turns
   // it off (read at finalize).
   static constexpr int kPredSlots = 8;
+  static_assert(kPredSlots == 8, "one -1 per slot below");
   int pcg_pred[2][kPredSlots] = {{-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1}};
   bool pcg_predict = true;
 

@@ -1012,7 +1012,11 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
     }
     return ONSAS_OK;
   };
-  int ahead = expect > kFirstIters ? expect - kFirstIters : 0;
+  // at most kMaxAhead (and maxiter) iterations before the first check, which
+  // bounds what an outlying count (a solve that ran to maxiter) can queue
+  constexpr int kMaxAhead = 256;
+  int ahead = expect > kFirstIters ? std::min(expect, maxiter) - kFirstIters : 0;
+  ahead = std::min(std::max(ahead, 0), kMaxAhead);
   ahead += ahead & 1;
   const int more = expect >= 0 && amg ? kFirstIters : chunk_iters;
   const int nsample = kFirstIters;
