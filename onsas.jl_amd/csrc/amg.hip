@@ -476,10 +476,15 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
     const int32_t* __restrict__ sell_ptr, const int32_t* __restrict__ slice_rounds, int64_t rstride,
     const int32_t* __restrict__ scol, const VT* __restrict__ sval, const KT* __restrict__ v,
     const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out,
-    const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st) {
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st,
+    double* __restrict__ prz) {
+  const int64_t s0 = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (s >= nslices) return;
+  // with prz every wave of the workgroup reaches its barrier: a wave past the
+  // last slice reads that slice's header, runs no rounds and writes nothing
+  if (s0 >= nslices && !prz) return;
+  const bool live = s0 < nslices;
+  const int64_t s = live ? s0 : nslices - 1;
   // the epilogue's row operands depend on nothing but the row: loaded first,
   // with the stop flag, so their latency hides under the product's
   const int64_t i = kSellRows * s + lane % kSellRows;
@@ -499,8 +504,8 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
   }
   const int32_t su = __builtin_amdgcn_readfirstlane((int32_t)s);  // wave-uniform: scalar loads
   const int32_t b0 = sell_ptr[su];
-  const int32_t rounds = slice_rounds[su];
-  if (pcg_stopped(st)) return;
+  const int32_t rounds = live ? slice_rounds[su] : 0;
+  if (pcg_stopped(st)) return;  // the same for every wave of the launch
   const uint32_t rs = (uint32_t)rstride;
   const int32_t* cp = scol + b0 + lane;
   const VT* vp = sval + 9u * (uint32_t)b0 + lane;
@@ -518,6 +523,29 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
     a0 += __shfl_xor(a0, o, 64);
     a1 += __shfl_xor(a1, o, 64);
     a2 += __shfl_xor(a2, o, 64);
+  }
+  if (prz) {
+    // with x_out: also b.x_out over the workgroup's rows (the PCG's r.z when b
+    // is its residual), one partial per workgroup, the 4 wave sums in order
+    __shared__ double wz[4];
+    double bz = 0;
+    if (live && lane < kSellRows && i < nn) {
+      KT r[3] = {bv[0] - a0, bv[1] - a1, bv[2] - a2};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const KT vq = vv3[q];
+        const KT d = c1 * vq + c2 * (dm[3 * q] * r[0] + dm[3 * q + 1] * r[1] + dm[3 * q + 2] * r[2]);
+        const KT xo = xb[q] + vq + d;
+        x_out[3 * i + q] = xo;
+        bz += (double)bv[q] * (double)xo;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bz += __shfl_xor(bz, o, 64);
+    if (lane == 0) wz[threadIdx.x >> 6] = bz;
+    __syncthreads();
+    if (threadIdx.x == 0) prz[blockIdx.x] = ((wz[0] + wz[1]) + wz[2]) + wz[3];
+    return;
   }
   if (lane >= kSellRows || i >= nn) return;
   KT r[3] = {-a0, -a1, -a2};
@@ -1306,6 +1334,7 @@ int amg_build(Ctx* c) {
   A.ldc = (A.nc + 3) & ~3;
   ONSAS_TRY(alloc(c, &A.d_Ainv, c->kt * (size_t)A.nc * A.ldc));
   ONSAS_TRY(alloc(c, (void**)&A.d_part, sizeof(double) * 1024));
+  ONSAS_TRY(alloc(c, (void**)&A.d_rzs, sizeof(double) * 2 * (size_t)c->nslices));
   A.built = true;
   A.stale = true;
   return ONSAS_OK;
@@ -1335,14 +1364,15 @@ void launch_sell6(const AmgLevel& L, const void* A6, const KT* v, const KT* b, K
     launch_sell6_r<KT, VT, 8>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
 }
 
-// r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels)
+// r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels);
+// prz (fine level, with b and x_out and without r_out / d_out): b.x_out per slice
 // approx: the product's result only feeds a smoothing update; on the fine
 // level in FP64 mode (degree-2 smoothing) it may use the FP32 copy of the
 // tangent.  Coarse levels with a sliced-ELL copy use it for every product.
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
-             hipEvent_t eb = nullptr, bool approx = false) {
+             hipEvent_t eb = nullptr, bool approx = false, double* prz = nullptr) {
   hipStream_t s = c->stream;
   if (l == 0) {
     using ST = typename SmootherType<KT>::type;
@@ -1352,17 +1382,17 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
       hipLaunchKernelGGL((k_sell_cheb<KT, ST>), grid, dim3(256), 0, s, c->d_sell_ptr,
                          c->d_slice_rounds, rstride, c->d_sell_col, (const ST*)c->amg.d_Ks, v, b,
                          r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                         c->nslices, c->nn, st);
+                         c->nslices, c->nn, st, prz);
     else if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
       hipExtLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, ea, eb, 0, c->d_sell_ptr,
                             c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
                             r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                            c->nslices, c->nn, st);
+                            c->nslices, c->nn, st, prz);
     else
       hipLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, c->d_sell_ptr,
                          c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
                          r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                         c->nslices, c->nn, st);
+                         c->nslices, c->nn, st, prz);
   } else {
     const AmgLevel& L = c->amg.lev[l];
     using CT = typename CoarseSmootherType<KT>::type;
@@ -1491,7 +1521,8 @@ void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, 
 }
 
 template <typename KT>
-int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hipEvent_t* ev) {
+int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hipEvent_t* ev,
+             double* prz) {
   auto evs = [&](int l, int j) { return l == 0 && ev ? ev[j] : nullptr; };
   Amg& A = c->amg;
   hipStream_t s = c->stream;
@@ -1554,7 +1585,7 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     if (deg1) {
       // z = x + D^-1 (b - A x) / theta: the degree-1 step in the product's epilogue
       op_cheb<KT>(c, l, x, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr, z, st, evs(l, 2),
-                  evs(l, 3));
+                  evs(l, 3), false, l == 0 ? prz : nullptr);
     } else {
       // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
       op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
@@ -1572,10 +1603,12 @@ int amg_numeric(Ctx* c) {
 }
 
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
-                  hipEvent_t* ev) {
+                  hipEvent_t* ev, double* prz) {
+  if (prz && c->amg.fine_degree != 1)
+    return set_err(c, ONSAS_ERR_STATE, "r.z partials come from the degree-1 fine smoothing only");
   return c->precision == ONSAS_PRECISION_FP64
-             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready, ev)
-             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev);
+             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready, ev, prz)
+             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev, prz);
 }
 
 int amg_prepare(Ctx* c) {
@@ -1736,7 +1769,8 @@ void amg_free(Ctx* c) {
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
                     (void*)L.d_s6_row, L.d_A6s})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks})
+  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks,
+                  (void*)A.d_rzs})
     if (p) (void)hipFree(p);
   A = Amg{};
 }

@@ -173,6 +173,7 @@ struct Amg {
   int ldc = 0;                        // row length of d_Ainv (nc rounded up to 4)
   void* d_Ainv = nullptr;             // nc x ldc KT, zero padded
   double* d_part = nullptr;           // power-iteration partial sums
+  double* d_rzs = nullptr;            // 2 x nslices r.z partials of the PCG (degree-1 fine level)
 };
 
 struct PcgState {     // device-resident scalars of the PCG iteration
@@ -359,8 +360,10 @@ int amg_numeric(Ctx* c);                                        // Galerkin, inv
 // the buffer amg_fine_smoother names (fused into the caller's kernel)
 // ev (optional): 4 events, start/stop pairs stamped by the dispatches of the
 // cycle's two exact fine-level products (r = b - A x), in cycle order
+// prz (optional, degree-1 fine smoothing only): one r.z partial per fine-level
+// slice, b.x of the cycle's input and output, written by its last product
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
-                  hipEvent_t* ev = nullptr);
+                  hipEvent_t* ev = nullptr, double* prz = nullptr);
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
 int amg_prepare(Ctx* c);  // refresh the smoother copy of the tangent (outside graphs)
 void amg_free(Ctx* c);

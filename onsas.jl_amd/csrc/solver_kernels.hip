@@ -53,16 +53,36 @@ __device__ inline double sum_parts(const double* __restrict__ p, int np, double*
 }
 
 // Sums of up to three partial arrays in one pass (one L2 round trip, one LDS
-// exchange).  p2 may be null (its sum is then 0).  sh needs 3 * T / 64 doubles.
+// exchange): p0 and p2 of n02 entries, p1 of n1.  p2 may be null (its sum is
+// then 0).  Each array's sum has the same order as when summed alone (thread
+// t adds entries t, t + T, ...; then a fixed tree).  sh needs 3 * T / 64 doubles.
 template <int T = NT>
 __device__ inline void sum_parts3(const double* __restrict__ p0, const double* __restrict__ p1,
-                                  const double* __restrict__ p2, int np, double* sh, double& s0,
-                                  double& s1, double& s2) {
+                                  const double* __restrict__ p2, int n02, int n1, double* sh,
+                                  double& s0, double& s1, double& s2) {
   double v0 = 0, v1 = 0, v2 = 0;
-  for (int i = threadIdx.x; i < np; i += T) {
-    v0 += p0[i];
-    v1 += p1[i];
-    if (p2) v2 += p2[i];
+  const int n = n02 > n1 ? n02 : n1;
+  // four strides per pass: the loads first (in flight together), then the
+  // adds in index order
+  constexpr int U = 4;
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * T) {
+    double x0[U], x1[U], x2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * T;
+      x0[u] = i < n02 ? p0[i] : 0.0;
+      x2[u] = (p2 && i < n02) ? p2[i] : 0.0;
+      x1[u] = i < n1 ? p1[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * T;
+      if (i < n02) {
+        v0 += x0[u];
+        v2 += x2[u];
+      }
+      if (i < n1) v1 += x1[u];
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -340,12 +360,12 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
                                                    const double* __restrict__ prz_prev,
                                                    const double* __restrict__ prr,
                                                    double* __restrict__ ppq, PcgState* __restrict__ st,
-                                                   int np, int64_t chunk, int64_t nn) {
+                                                   int np, int nrz, int64_t chunk, int64_t nn) {
   __shared__ double sh[3 * kSpmvThreads / 64];
   if (st->done | st->breakdown) return;
   const int k = st->k;
   double rz, rr, rzp;
-  sum_parts3<kSpmvThreads>(prz_cur, prr, k > 0 ? prz_prev : nullptr, np, sh, rz, rr, rzp);
+  sum_parts3<kSpmvThreads>(prz_cur, prr, k > 0 ? prz_prev : nullptr, nrz, np, sh, rz, rr, rzp);
   const double beta = k > 0 ? rz / rzp : 0.0;
   const bool conv = rr <= st->tol2;
   if (conv || k >= st->maxiter) {
@@ -425,7 +445,7 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B(const KT* __restrict__ p, const
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
   double rz, pq, unused;
-  sum_parts3(prz_cur, ppq, nullptr, np, sh, rz, pq, unused);
+  sum_parts3(prz_cur, ppq, nullptr, np, np, sh, rz, pq, unused);
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
@@ -539,7 +559,7 @@ __global__ __launch_bounds__(NT) void k_warm_init(
     int64_t chunk, int64_t nn) {
   __shared__ double sh[3 * NT / 64];
   double tb, tt, unused;
-  sum_parts3(ptb, ptt, nullptr, np, sh, tb, tt, unused);
+  sum_parts3(ptb, ptt, nullptr, np, np, sh, tb, tt, unused);
   const double a = (tt > 0 && isfinite(tb) && isfinite(tt)) ? tb / tt : 0.0;
   int64_t r0, r1;
   wg_rows(np, chunk, nn, r0, r1);
@@ -587,12 +607,12 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, con
                                                      const double* __restrict__ ppq,
                                                      double* __restrict__ prr, PcgState* __restrict__ st,
                                                      const KT* __restrict__ Dinv, KT s,
-                                                     KT* __restrict__ d0, int np, int64_t chunk,
-                                                     int64_t nn) {
+                                                     KT* __restrict__ d0, int np, int nrz,
+                                                     int64_t chunk, int64_t nn) {
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
   double rz, pq, unused;
-  sum_parts3(prz_cur, ppq, nullptr, np, sh, rz, pq, unused);
+  sum_parts3(prz_cur, ppq, nullptr, nrz, np, sh, rz, pq, unused);
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
@@ -836,7 +856,7 @@ __global__ __launch_bounds__(T) void k_pcg_check(const double* __restrict__ prr,
   __shared__ double sh[3 * T / 64];
   if (st->done | st->breakdown) return;
   double unused0, rr, unused2;
-  sum_parts3<T>(prr, prr, nullptr, np, sh, unused0, rr, unused2);
+  sum_parts3<T>(prr, prr, nullptr, np, np, sh, unused0, rr, unused2);
   const int k = st->k;  // iterations completed
   const bool conv = rr <= st->tol2;
   if (threadIdx.x == 0 && (conv || k >= st->maxiter)) {
@@ -847,11 +867,26 @@ __global__ __launch_bounds__(T) void k_pcg_check(const double* __restrict__ prr,
   }
 }
 
+// r.z partials of parity par: with the multigrid's degree-1 fine smoothing one
+// per workgroup (4 slices) of the cycle's last fine-level product, which
+// writes them; otherwise one per reduction workgroup
+inline bool rz_from_cycle(const Ctx* c) {
+  return c->lin.precond == ONSAS_PRECOND_AMG && c->amg.fine_degree == 1;
+}
+inline double* rz_parts(Ctx* c, int par) {
+  return rz_from_cycle(c) ? c->amg.d_rzs + (size_t)par * c->nslices
+                          : part(c, par == 0 ? kPartRZ0 : kPartRZ1);
+}
+inline int rz_count(const Ctx* c) {
+  return rz_from_cycle(c) ? (int)((c->nslices + 3) / 4) : c->np;  // per 4-slice workgroup
+}
+
 template <typename KT>
 void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEvent_t* evc = nullptr) {
   hipStream_t s = c->stream;
   KT* P[2] = {(KT*)c->d_pp0, (KT*)c->d_pp1};
-  double* RZ[2] = {part(c, kPartRZ0), part(c, kPartRZ1)};
+  double* RZ[2] = {rz_parts(c, 0), rz_parts(c, 1)};
+  const int nrz = rz_count(c);
   PcgState* st = &c->d_sc->pcg;
   const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
   if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
@@ -859,24 +894,26 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
                           c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
                           (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par],
                           (KT*)c->d_pq, RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ),
-                          st, c->np, c->chunk, c->nn);
+                          st, c->np, nrz, c->chunk, c->nn);
   else
     hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
                        c->d_slice_rounds, rstride, c->d_sell_col,
                        (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
-                       RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np,
+                       RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, nrz,
                        c->chunk, c->nn);
   if (c->lin.precond == ONSAS_PRECOND_AMG) {
     KT* d0 = nullptr;
     const double s0 = amg_fine_smoother(c, (void**)&d0);
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
-                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, c->chunk, c->nn);
+                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, nrz, c->chunk, c->nn);
     hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
                        part(c, kPartRR), st, c->np);
-    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc);
-    hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
-                       (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
+    const bool fused = rz_from_cycle(c);
+    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc, fused ? RZ[1 - par] : nullptr);
+    if (!fused)
+      hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
+                         (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
     return;
   }
   hipLaunchKernelGGL(k_pcg_B<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
@@ -966,9 +1003,13 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
     // preconditioner application of r_0 (same rule as the first k_pcg_A)
     hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
                        part(c, kPartRR), &c->d_sc->pcg, np);
-    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, false));
-    hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
-                       (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk, c->nn);
+    const bool fused = rz_from_cycle(c);
+    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, false, nullptr,
+                            fused ? rz_parts(c, 0) : nullptr));
+    if (!fused)
+      hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
+                         (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk,
+                         c->nn);
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   // ONSAS_PCG_GRAPH=0 replays the chunks as direct launches (same kernels, same
