@@ -985,15 +985,16 @@ int upload_kt(Ctx* c, void** d, const std::vector<double>& h) {
 
 // Rows per sliced-ELL slice of a coarse level.  Fewer rows per wave split a
 // row over more lanes (fewer dependent index -> gather rounds, more waves in
-// flight) at the price of more padding; on the 64^3 cube 8, 4 and 2 measured
-// the same (these launches are bound by latency, not bytes), so 8 (the least
-// padding).  ONSAS_SELL6_RPW (8, 4 or 2) overrides.
+// flight) at the price of more padding; these launches are bound by latency,
+// not bytes.  On the 64^3 cube 4 rows per wave gave the shortest Newton solve
+// (1 ms less than 8 over five alternating pairs; 2 was slower in FP64 mode).
+// ONSAS_SELL6_RPW (8, 4 or 2) overrides.
 int sell6_rows_per_wave() {
   if (const char* e = std::getenv("ONSAS_SELL6_RPW")) {
     const int r = std::atoi(e);
     if (r == 2 || r == 4 || r == 8) return r;
   }
-  return 8;
+  return 4;
 }
 
 // greedy aggregation of a node graph (CSR, self included)
