@@ -180,6 +180,7 @@ struct PcgState {     // device-resident scalars of the PCG iteration
   double tol2;        // (rtol ||b||)^2
   double rr_last;     // ||r_k||^2 at the last convergence check
   double bb;          // ||b||^2
+  double rzs[2];      // r.z as k_pcg_A summed it, by iteration parity (k & 1)
   int32_t k;          // iterations completed
   int32_t done;       // 1 once converged or k reached maxiter
   int32_t iters;      // iteration count when done

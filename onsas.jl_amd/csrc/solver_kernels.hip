@@ -360,12 +360,19 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
                                                    const double* __restrict__ prz_prev,
                                                    const double* __restrict__ prr,
                                                    double* __restrict__ ppq, PcgState* __restrict__ st,
-                                                   int np, int nrz, int64_t chunk, int64_t nn) {
+                                                   int np, int nrz, int64_t chunk, int64_t nn,
+                                                   int par) {
   __shared__ double sh[3 * kSpmvThreads / 64];
   if (st->done | st->breakdown) return;
   const int k = st->k;
-  double rz, rr, rzp;
-  sum_parts3<kSpmvThreads>(prz_cur, prr, k > 0 ? prz_prev : nullptr, nrz, np, sh, rz, rr, rzp);
+  // r.z of this iteration from its partials; that of the previous one as the
+  // previous launch summed it (kept by parity, so no workgroup reads the slot
+  // being written)
+  double rz, rr, unused;
+  (void)prz_prev;
+  sum_parts3<kSpmvThreads>(prz_cur, prr, nullptr, nrz, np, sh, rz, rr, unused);
+  const double rzp = k > 0 ? st->rzs[par ^ 1] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rzs[par] = rz;
   const double beta = k > 0 ? rz / rzp : 0.0;
   const bool conv = rr <= st->tol2;
   if (conv || k >= st->maxiter) {
@@ -608,11 +615,15 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, con
                                                      double* __restrict__ prr, PcgState* __restrict__ st,
                                                      const KT* __restrict__ Dinv, KT s,
                                                      KT* __restrict__ d0, int np, int nrz,
-                                                     int64_t chunk, int64_t nn) {
+                                                     int64_t chunk, int64_t nn, int par) {
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
-  double rz, pq, unused;
-  sum_parts3(prz_cur, ppq, nullptr, nrz, np, sh, rz, pq, unused);
+  // r.z as k_pcg_A summed it (same value bit for bit); p.q from its partials
+  double pq, unused0, unused2;
+  (void)prz_cur;
+  (void)nrz;
+  sum_parts3(ppq, ppq, nullptr, 0, np, sh, unused0, pq, unused2);
+  const double rz = st->rzs[par];
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
@@ -894,19 +905,19 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
                           c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
                           (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par],
                           (KT*)c->d_pq, RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ),
-                          st, c->np, nrz, c->chunk, c->nn);
+                          st, c->np, nrz, c->chunk, c->nn, par);
   else
     hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
                        c->d_slice_rounds, rstride, c->d_sell_col,
                        (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
                        RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, nrz,
-                       c->chunk, c->nn);
+                       c->chunk, c->nn, par);
   if (c->lin.precond == ONSAS_PRECOND_AMG) {
     KT* d0 = nullptr;
     const double s0 = amg_fine_smoother(c, (void**)&d0);
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
-                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, nrz, c->chunk, c->nn);
+                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, nrz, c->chunk, c->nn, par);
     hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
                        part(c, kPartRR), st, c->np);
     const bool fused = rz_from_cycle(c);
