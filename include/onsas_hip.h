@@ -38,7 +38,7 @@ typedef struct onsas_ctx onsas_ctx;
 
 /* ---- enums --------------------------------------------------------------- */
 #define ONSAS_PRECISION_MIXED 0 /* FP32 tangent + FP32 PCG, FP64 f_int/residual/norms/U */
-#define ONSAS_PRECISION_FP64 1  /* everything FP64                                      */
+#define ONSAS_PRECISION_FP64 1  /* everything FP64, the multigrid's stored values too   */
 
 #define ONSAS_MAT_SVK 1        /* src/Materials/SVKMaterial.jl          params (lambda, G) */
 #define ONSAS_MAT_NEOHOOKEAN 2 /* src/Materials/NeoHookeanMaterial.jl   params (K, G)      */
@@ -48,7 +48,7 @@ typedef struct onsas_ctx onsas_ctx;
 #define ONSAS_STRAIN_GREEN 1   /* GreenStrain (src/Entities/Trusses.jl:44)              */
 
 #define ONSAS_LINSOLVE_AUTO 0  /* PCG; GMRES when PCG breaks down (p'Kp <= 0)        */
-#define ONSAS_LINSOLVE_PCG 1   /* PCG only (block-Jacobi)                             */
+#define ONSAS_LINSOLVE_PCG 1   /* PCG only (preconditioner: ONSAS_PRECOND_*)          */
 #define ONSAS_LINSOLVE_GMRES 2 /* restarted GMRES only (block-Jacobi, right precond.) */
 
 #define ONSAS_PRECOND_BLOCK_JACOBI 0 /* 3x3 nodal block-Jacobi                          */
@@ -156,7 +156,7 @@ int onsas_add_tet_group(onsas_ctx* ctx, int64_t ne, const int32_t* conn, int32_t
 /* A group of Truss elements (src/Entities/Trusses.jl:54-99) with an
  * AbstractHyperElasticMaterial of elasticity modulus E
  * (elasticity_modulus, SVKMaterial.jl:75-78).  conn: 2*ne; area: ne values
- * (area(cross_section), src/CrossSections/*). */
+ * (area(cross_section), e.g. src/CrossSections/Circles.jl:20). */
 int onsas_add_truss_group(onsas_ctx* ctx, int64_t ne, const int32_t* conn,
                           int32_t strain_model, double E, const double* area, int32_t* group);
 
@@ -246,7 +246,7 @@ int onsas_amg_get_operator(onsas_ctx* ctx, int32_t level, int32_t* row_ptr, int3
                            double* vals);
 /* Prolongator from level l+1 to level l, block CSR over level-l nodes with
  * bs x 6 blocks: p_ptr (n+1), p_col (nnz_p), vals (bs*6 per block, row-major;
- * FP64 copy of the stored FP16 (mixed) / FP32 (FP64 mode) values). */
+ * the stored FP16 values widened (mixed) or the stored FP64 values (FP64 mode)). */
 int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int32_t* p_col,
                               double* vals);
 /* Measurement (no counterpart in the reference): `launches` (2..10000)
@@ -257,6 +257,14 @@ int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int
  * trace reports it.  Needs the hierarchy (onsas_amg_levels > 0); scribbles
  * on the cycle's work vectors only. */
 int onsas_time_cycle_products(onsas_ctx* ctx, int32_t launches, double* ms_per_launch);
+/* The same measurement for any smoothed level (no counterpart in the
+ * reference): level 0 as onsas_time_cycle_products; a level >= 1 stored in
+ * sliced ELL cycles through its four products of the V-cycle (x = d0 + d1,
+ * r = b - A x, r and d0 = D^-1 r / theta, x += d0 + d1) on that level's work
+ * vectors.  Levels without a sliced-ELL operator and the coarsest level are
+ * rejected (ONSAS_ERR_ARG). */
+int onsas_time_level_products(onsas_ctx* ctx, int32_t level, int32_t launches,
+                              double* ms_per_launch);
 /* y = K x with the assembled tangent (n values each). */
 int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y);
 

@@ -65,36 +65,24 @@ __device__ inline T pick6(const T (&a)[6], int u) {
   return r;
 }
 
-// Value type of the smoother copy of the fine tangent: FP32 in FP64 mode (half
-// the bytes of the smoothing-only products).  Mixed mode has no copy: its
-// tangent is FP32 already, and a BF16 copy measured no faster overall (its
-// products are bound by load instructions, not bytes, and it costs a
-// conversion per assembly).
-template <typename KT>
-struct SmootherType {
-  using type = KT;
-};
-template <>
-struct SmootherType<double> {
-  using type = float;
-};
-
 template <typename KT, typename VT>
 __device__ inline KT ldv(VT x) {
   return (KT)x;
 }
 
-// Value type of the smoothing-only copy of the sliced-ELL coarse operators: BF16
-// (stored as uint16, round to nearest even; BF16 for the FP32 range, coarse
-// entries can exceed FP16's) in mixed mode, FP32 in FP64 mode.  These products
-// load 16 bytes per lane, so half the bytes is half the traffic.
+// Value type of the sliced-ELL copy of a large coarse operator, which every
+// product of that level's cycle reads: BF16 (stored as uint16, round to nearest
+// even; BF16 for the FP32 range, coarse entries can exceed FP16's) in mixed
+// mode, FP64 in FP64 mode (which keeps every stored value in FP64).  These
+// products load 16 bytes per lane in mixed mode, so half the bytes is half the
+// traffic.
 template <typename KT>
 struct CoarseSmootherType {
   using type = uint16_t;
 };
 template <>
 struct CoarseSmootherType<double> {
-  using type = float;
+  using type = double;
 };
 
 template <>
@@ -114,7 +102,7 @@ __device__ inline uint16_t to_st<uint16_t, float>(float x) {
 }
 
 // Value type of the smoothed prolongator P (and its P^T copy): FP16 in mixed
-// mode, FP32 in FP64 mode.  P's entries are O(1) (orthonormalised modes); W,
+// mode, FP64 in FP64 mode.  P's entries are O(1) (orthonormalised modes); W,
 // the coarse operators, restriction and prolongation all use the same rounded
 // values, so the coarse operator stays the exact Galerkin product of the P the
 // cycle applies.
@@ -124,16 +112,10 @@ struct ProlongType {
 };
 template <>
 struct ProlongType<double> {
-  using type = float;
+  using type = double;
 };
 template <typename KT>
 using PT_t = typename ProlongType<KT>::type;
-
-__global__ void k_to_smoother_f64(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = (float)s[i];
-}
 
 // Sum of a 6-vector over the 64 lanes of a wave (every lane gets the total).
 template <typename KT>
@@ -200,7 +182,9 @@ __global__ __launch_bounds__(256) void k_sa_prolongator(
     double ds = 0;
 #pragma unroll
     for (int m = 0; m < BS; ++m) ds += (double)d[BS * r + m] * s[m];
-    const PT_t<KT> v = (PT_t<KT>)(float)((own ? (double)ti[6 * r] : 0.0) - w * ds);
+    const double pv = (own ? (double)ti[6 * r] : 0.0) - w * ds;
+    // FP16 (mixed): rounded through FP32; FP64: stored as computed
+    const PT_t<KT> v = std::is_same<PT_t<KT>, double>::value ? (PT_t<KT>)pv : (PT_t<KT>)(float)pv;
     o[6 * r] = v;
     ot[6 * r] = v;
   }
@@ -606,14 +590,35 @@ __global__ __launch_bounds__(256) void k_bsr6_cheb(
   if (x_out) x_out[6 * I + l] = (x_base ? x_base[6 * I + l] : KT(0)) + vu + d;
 }
 
+// Value layout of one round of a sliced-ELL level (36 x 64 values: 36 per lane):
+// 16-byte chunks of kVPC = 16 / sizeof(VT) values, lane-interleaved (chunk ch
+// of lane l at (64 ch + l) kVPC), kFull whole chunks, then the kRem values left
+// (BF16: 36 = 4 x 8 + 4) as one narrower chunk at 64 kFull kVPC + kRem l.  Each
+// value load of a wave then reads 64 x 16 contiguous bytes (the last one of a
+// BF16 round 64 x 8).
+template <typename VT>
+struct Sell6Layout {
+  static constexpr int kVPC = 16 / (int)sizeof(VT);
+  static constexpr int kFull = 36 / kVPC;
+  static constexpr int kRem = 36 - kFull * kVPC;
+  static constexpr int64_t kRound = 36 * 64;
+  __host__ __device__ static int64_t offset(int e, int l) {
+    return e < kFull * kVPC ? (int64_t)((e / kVPC) * 64 + l) * kVPC + e % kVPC
+                            : (int64_t)kFull * kVPC * 64 + (int64_t)l * kRem + (e - kFull * kVPC);
+  }
+};
+
 // Large levels >= 1 on their sliced-ELL copy: one wave per slice of RPW rows,
-// 64 / RPW lanes per row (lane l serves row l % RPW); every value load of a
-// round reads 64 x 16 contiguous bytes.  The column indices of the next round
-// are fetched before this round's products, so the index -> x gather chain of
-// one round overlaps the value loads of the previous one.  After the reduction
-// over the lanes of a row, lanes l / RPW < 6 finish that component of row
-// s6_row[RPW s + l % RPW] (rows sorted by length within windows; -1: none).
-template <typename KT, typename VT, int RPW>
+// 64 / RPW lanes per row (lane l serves row l % RPW).  A lane issues the loads
+// of UR rounds at once (column indices, the 36 values of each round, then the
+// gathers of x), with the round index clamped to the slice's last round, before
+// any product; the products of rounds past the end are skipped (a wave-uniform
+// test), so every sum runs over the rounds and the 36 values in order as a
+// one-round loop would.  A slice of up to UR rounds thus costs one index ->
+// gather chain instead of one per round.  After the reduction over the lanes of
+// a row, lanes l / RPW < 6 finish that component of row s6_row[RPW s + l % RPW]
+// (rows sorted by length within windows; -1: none).
+template <typename KT, typename VT, int RPW, int UR>
 __global__ __launch_bounds__(256) void k_sell6_cheb(
     const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
     const int32_t* __restrict__ s6_row, const VT* __restrict__ A6, const KT* __restrict__ v,
@@ -621,6 +626,8 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
     const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
     KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
   static_assert(RPW == 2 || RPW == 4 || RPW == 8, "6 components need >= 8 lanes per row");
+  static_assert(UR >= 1 && UR <= 4, "1 to 4 rounds in flight (registers: 36 values each)");
+  using LY = Sell6Layout<VT>;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (s >= nslices) return;
@@ -643,22 +650,42 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
   const int32_t q0 = s6_ptr[su], q1 = s6_ptr[su + 1];
   if (pcg_stopped(st)) return;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  int32_t col = q0 < q1 ? s6_col[64 * (int64_t)q0 + l] : 0;
-  for (int32_t q = q0; q < q1; ++q) {
-    const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)col, 8);
-    KT xv[6];
+  for (int32_t qb = q0; qb < q1; qb += UR) {
+    int32_t cj[UR];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) xv[c] = xj[c];
-    col = q + 1 < q1 ? s6_col[64 * (int64_t)(q + 1) + l] : 0;
-    const VT* a = A6 + 36 * 64 * (int64_t)q + 4 * l;
+    for (int k = 0; k < UR; ++k) cj[k] = s6_col[64 * (int64_t)min(qb + k, q1 - 1) + l];
+    VT av[UR][36];
 #pragma unroll
-    for (int ch = 0; ch < 9; ++ch) {
-      const VT* ap = (const VT*)__builtin_assume_aligned(a + 256 * ch, 4 * sizeof(VT));
+    for (int k = 0; k < UR; ++k) {  // unconditional loads (clamped rounds)
+      const VT* a = A6 + LY::kRound * (int64_t)min(qb + k, q1 - 1);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int e = 4 * ch + j;
-        acc[e / 6] += ldv<KT>(ap[j]) * xv[e % 6];
+      for (int ch = 0; ch < LY::kFull; ++ch) {
+        const VT* ap = (const VT*)__builtin_assume_aligned(a + LY::offset(ch * LY::kVPC, l), 16);
+#pragma unroll
+        for (int j = 0; j < LY::kVPC; ++j) av[k][ch * LY::kVPC + j] = ap[j];
       }
+      if constexpr (LY::kRem > 0) {
+        const VT* ap = (const VT*)__builtin_assume_aligned(
+            a + LY::offset(LY::kFull * LY::kVPC, l), LY::kRem * sizeof(VT));
+#pragma unroll
+        for (int j = 0; j < LY::kRem; ++j) av[k][LY::kFull * LY::kVPC + j] = ap[j];
+      }
+    }
+    KT xv[UR][6];
+#pragma unroll
+    for (int k = 0; k < UR; ++k) {
+      const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)cj[k], 8);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) xv[k][c] = xj[c];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every load of the pass issued before the products
+    // a clamped round past the slice's end multiplies by exact zeros: the sum
+    // is unchanged, and no branch lets the loads sink below a test
+#pragma unroll
+    for (int k = 0; k < UR; ++k) {
+      const bool ok = qb + k < q1;
+#pragma unroll
+      for (int e = 0; e < 36; ++e) acc[e / 6] += ldv<KT>(av[k][e]) * (ok ? xv[k][e % 6] : KT(0));
     }
   }
 #pragma unroll
@@ -680,7 +707,7 @@ __global__ __launch_bounds__(256) void k_sell6_cheb(
   if (x_out) x_out[6 * I + u] = xb + vu + d;
 }
 
-// Block-CSR -> sliced-ELL values: one thread per (round slot, value).
+// Block-CSR -> sliced-ELL values (Sell6Layout): one thread per (round slot, value).
 template <typename KT>
 __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restrict__ A,
                            typename CoarseSmootherType<KT>::type* __restrict__ A6s, int64_t nslots) {
@@ -693,7 +720,7 @@ __global__ void k_to_sell6(const int32_t* __restrict__ s6_src, const KT* __restr
   const int l = (int)(slot & 63);
   const int32_t k = s6_src[slot];
   const KT v = k >= 0 ? A[36 * (int64_t)k + e] : KT(0);
-  A6s[((9 * q + e / 4) * 64 + l) * 4 + (e & 3)] = to_st<ST>(v);
+  A6s[Sell6Layout<ST>::kRound * q + Sell6Layout<ST>::offset(e, l)] = to_st<ST>(v);
 }
 
 // y = s D^-1 v, one thread per (node, component).
@@ -997,6 +1024,17 @@ int sell6_rows_per_wave() {
   return 4;
 }
 
+// Rounds of a sliced-ELL coarse level whose loads a lane issues together
+// (k_sell6_cheb): 4 with BF16 values (128 VGPRs, 4 waves per SIMD), 2 with
+// FP64 values (36 doubles a round).  ONSAS_SELL6_ROUNDS (1, 2 or 4) overrides.
+int sell6_rounds_in_flight(bool f64) {
+  if (const char* e = std::getenv("ONSAS_SELL6_ROUNDS")) {
+    const int r = std::atoi(e);
+    if (r == 1 || r == 2 || r == 4) return r;
+  }
+  return f64 ? 2 : 4;
+}
+
 // greedy aggregation of a node graph (CSR, self included)
 int64_t aggregate(const std::vector<int32_t>& rp, const std::vector<int32_t>& cl, int64_t n,
                   std::vector<int32_t>& agg) {
@@ -1251,6 +1289,7 @@ int amg_build(Ctx* c) {
       const int rpw = sell6_rows_per_wave();
       const int lpr = 64 / rpw;
       Lc.rpw6 = rpw;
+      Lc.ur6 = sell6_rounds_in_flight(f64);
       for (int64_t r0 = 0; r0 < na; r0 += rpw) {
         int32_t width = 0;
         for (int r = 0; r < rpw; ++r) {
@@ -1294,7 +1333,7 @@ int amg_build(Ctx* c) {
     ONSAS_TRY(upload(c, &F.d_gl_ptr, gl_ptr));
     ONSAS_TRY(upload(c, &F.d_gl_p, gl_p));
     ONSAS_TRY(upload(c, &F.d_gl_w, gl_w));
-    const size_t pbytes = f64 ? sizeof(float) : sizeof(_Float16);  // P's value type
+    const size_t pbytes = f64 ? sizeof(double) : sizeof(_Float16);  // P's value type
     ONSAS_TRY(alloc(c, &F.d_P, pbytes * bs * 6 * (size_t)nnz_p));
     ONSAS_TRY(alloc(c, &F.d_PT, pbytes * bs * 6 * (size_t)nnz_p));
     ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)nnz_w));
@@ -1307,7 +1346,7 @@ int amg_build(Ctx* c) {
       ONSAS_TRY(upload(c, &Lc.d_s6_col, s6_col));
       ONSAS_TRY(upload(c, &Lc.d_s6_src, s6_src));
       ONSAS_TRY(upload(c, &Lc.d_s6_row, s6_row));
-      ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(float) : sizeof(uint16_t)) * 36 * 64 *
+      ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(double) : sizeof(uint16_t)) * 36 * 64 *
                                         (size_t)Lc.nrounds6));
     }
     ONSAS_TRY(alloc(c, &Lc.d_Dinv, c->kt * 36 * (size_t)na));
@@ -1327,8 +1366,6 @@ int amg_build(Ctx* c) {
   }
   for (void** v : {&A.lev[0].d_x, &A.lev[0].d_r, &A.lev[0].d_d0, &A.lev[0].d_d1, &A.lev[0].d_ev})
     ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));
-  if (f64 && A.fine_degree == 2) ONSAS_TRY(alloc(c, &A.d_Ks, sizeof(float) * 9 * (size_t)c->nsell));
-  A.ks_stale = true;
   A.nc = 6 * (int)A.lev.back().n;
   ONSAS_TRY(alloc(c, (void**)&A.d_dense, sizeof(double) * (size_t)A.nc * A.nc));
   ONSAS_TRY(alloc(c, (void**)&A.d_gjc, sizeof(double) * (size_t)A.nc * kGJ));
@@ -1343,14 +1380,26 @@ int amg_build(Ctx* c) {
 
 namespace {
 
+template <typename KT, typename VT, int R, int UR>
+void launch_sell6_ru(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
+                     double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
+                     const PcgState* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R, UR>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
+                     L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const VT*)A6, v, b, r_out,
+                     (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6, L.n,
+                     st);
+}
+
 template <typename KT, typename VT, int R>
 void launch_sell6_r(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
                     double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
                     const PcgState* st, hipStream_t s) {
-  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
-                     L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const VT*)A6, v, b, r_out,
-                     (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6, L.n,
-                     st);
+  if (L.ur6 == 1)
+    launch_sell6_ru<KT, VT, R, 1>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
+  else if (L.ur6 == 2)
+    launch_sell6_ru<KT, VT, R, 2>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
+  else
+    launch_sell6_ru<KT, VT, R, 4>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
 }
 
 template <typename KT, typename VT>
@@ -1366,25 +1415,18 @@ void launch_sell6(const AmgLevel& L, const void* A6, const KT* v, const KT* b, K
 }
 
 // r = b - A_l v and the per-row Chebyshev epilogue (see the operator kernels);
-// prz (fine level, with b and x_out and without r_out / d_out): b.x_out per slice
-// approx: the product's result only feeds a smoothing update; on the fine
-// level in FP64 mode (degree-2 smoothing) it may use the FP32 copy of the
-// tangent.  Coarse levels with a sliced-ELL copy use it for every product.
+// prz (fine level, with b and x_out and without r_out / d_out): b.x_out per slice.
+// The fine level reads the tangent itself; coarse levels with a sliced-ELL copy
+// use it for every product.
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
-             hipEvent_t eb = nullptr, bool approx = false, double* prz = nullptr) {
+             hipEvent_t eb = nullptr, double* prz = nullptr) {
   hipStream_t s = c->stream;
   if (l == 0) {
-    using ST = typename SmootherType<KT>::type;
     const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
     const dim3 grid(nblocks(c->nslices, 4));
-    if (approx && !std::is_same<ST, KT>::value)
-      hipLaunchKernelGGL((k_sell_cheb<KT, ST>), grid, dim3(256), 0, s, c->d_sell_ptr,
-                         c->d_slice_rounds, rstride, c->d_sell_col, (const ST*)c->amg.d_Ks, v, b,
-                         r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                         c->nslices, c->nn, st, prz);
-    else if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
+    if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
       hipExtLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, ea, eb, 0, c->d_sell_ptr,
                             c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
                             r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
@@ -1397,9 +1439,9 @@ void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, doub
   } else {
     const AmgLevel& L = c->amg.lev[l];
     using CT = typename CoarseSmootherType<KT>::type;
-    // a level's products all use one operator (its sliced-ELL copy, BF16 /
-    // FP32 in FP64 mode), so its smoothing and residuals stay consistent and
-    // the cycle symmetric positive definite
+    // a level's products all use one operator (its sliced-ELL copy, BF16 in
+    // mixed mode), so its smoothing and residuals stay consistent and the
+    // cycle symmetric positive definite
     if (L.nslices6 > 0)
       launch_sell6<KT, CT>(L, L.d_A6s, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
     else
@@ -1546,9 +1588,8 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
                   evs(l, 0), evs(l, 1));
     } else {
       KT* x = (KT*)L.d_x;
-      // x = d0 + d1 (smoother copy at the fine level), then the exact r = b - A x
-      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
-                  nullptr, nullptr, true);
+      // x = d0 + d1, then r = b - A x
+      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st);
       op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 0), evs(l, 1));
     }
     AmgLevel& C = A.lev[l + 1];
@@ -1586,11 +1627,11 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     if (deg1) {
       // z = x + D^-1 (b - A x) / theta: the degree-1 step in the product's epilogue
       op_cheb<KT>(c, l, x, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr, z, st, evs(l, 2),
-                  evs(l, 3), false, l == 0 ? prz : nullptr);
+                  evs(l, 3), l == 0 ? prz : nullptr);
     } else {
-      // the exact r = b - A x and d0, then z = x + d0 + d1 (smoother copy)
+      // r = b - A x and d0, then z = x + d0 + d1
       op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, st, evs(l, 2), evs(l, 3));
-      op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st, nullptr, nullptr, true);
+      op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, z, st);
     }
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
@@ -1612,19 +1653,6 @@ int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_re
              : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev, prz);
 }
 
-int amg_prepare(Ctx* c) {
-  Amg& A = c->amg;
-  if (!A.ks_stale) return ONSAS_OK;
-  const int64_t n = 9 * c->nsell;
-  if (c->precision == ONSAS_PRECISION_FP64 && A.fine_degree == 2) {
-    hipLaunchKernelGGL(k_to_smoother_f64, dim3(2048), dim3(256), 0, c->stream,
-                       (const double*)c->d_Ksell, (float*)A.d_Ks, n);
-    ONSAS_HIP_TRY(c, hipGetLastError());
-  }
-  A.ks_stale = false;
-  return ONSAS_OK;
-}
-
 double amg_fine_smoother(Ctx* c, void** d0) {
   *d0 = c->amg.lev[0].d_d0;
   return cheb(c->amg.lev[0].lam).inv_theta;
@@ -1634,32 +1662,42 @@ double amg_fine_smoother(Ctx* c, void** d0) {
 
 // the hierarchy has values once a solve has run its numeric setup (lam_set)
 namespace {
-// the cycle's two fine-level products, alternating, on the last solve's
-// vectors (no stop flag), between two events
+// The cycle's products of level l, in cycle order, on the last solve's vectors
+// (no stop flag), between two events.  Level 0 alternates its two products
+// (degree 1: r = b - A d0 and z = d0 + D^-1 (b - A d0) / theta); a sliced-ELL
+// level >= 1 cycles through its four (x = d0 + d1, r = b - A x, r and d0 =
+// D^-1 r / theta, x += d0 + d1).
 template <typename KT>
-int amg_time_fine_products_t(Ctx* c, int launches, double* ms) {
+int amg_time_level_products_t(Ctx* c, int l, int launches, double* ms) {
   Amg& A = c->amg;
-  AmgLevel& L = A.lev[0];
+  AmgLevel& L = A.lev[l];
   const Cheb ch = cheb(L.lam);
   hipStream_t s = c->stream;
-  const KT* b = (const KT*)c->d_pr;
   hipEvent_t e0 = c->ev_spmv[2], e1 = c->ev_spmv[3];
+  KT *x = (KT*)L.d_x, *r = (KT*)L.d_r, *d0 = (KT*)L.d_d0, *d1 = (KT*)L.d_d1;
   ONSAS_HIP_TRY(c, hipEventRecord(e0, s));
   for (int i = 0; i < launches; ++i) {
-    if (A.fine_degree == 1) {  // r = b - A d0;  z = d0 + D^-1 (b - A d0) / theta
-      if (i & 1)
-        op_cheb<KT>(c, 0, (const KT*)L.d_d0, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr,
-                    (KT*)L.d_x, nullptr);
-      else
-        op_cheb<KT>(c, 0, (const KT*)L.d_d0, b, (KT*)L.d_r, 0.0, 0.0, nullptr, nullptr, nullptr,
-                    nullptr);
-    } else {  // r = b - A x;  r and d = D^-1 r / theta
-      if (i & 1)
-        op_cheb<KT>(c, 0, (const KT*)L.d_x, b, (KT*)L.d_r, 0.0, ch.inv_theta, (KT*)L.d_d1,
-                    nullptr, nullptr, nullptr);
-      else
-        op_cheb<KT>(c, 0, (const KT*)L.d_x, b, (KT*)L.d_r, 0.0, 0.0, nullptr, nullptr, nullptr,
-                    nullptr);
+    if (l == 0) {
+      const KT* b = (const KT*)c->d_pr;
+      if (A.fine_degree == 1) {
+        if (i & 1)
+          op_cheb<KT>(c, 0, d0, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr, x, nullptr);
+        else
+          op_cheb<KT>(c, 0, d0, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr);
+      } else {  // r = b - A x;  r and d = D^-1 r / theta
+        if (i & 1)
+          op_cheb<KT>(c, 0, x, b, r, 0.0, ch.inv_theta, d1, nullptr, nullptr, nullptr);
+        else
+          op_cheb<KT>(c, 0, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr);
+      }
+    } else {
+      const KT* b = (const KT*)L.d_b;
+      switch (i & 3) {
+        case 0: op_cheb<KT>(c, l, d0, b, nullptr, ch.c1, ch.c2, d1, nullptr, x, nullptr); break;
+        case 1: op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr); break;
+        case 2: op_cheb<KT>(c, l, x, b, r, 0.0, ch.inv_theta, d0, nullptr, nullptr, nullptr); break;
+        default: op_cheb<KT>(c, l, d0, r, nullptr, ch.c1, ch.c2, nullptr, x, x, nullptr); break;
+      }
     }
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
@@ -1672,12 +1710,18 @@ int amg_time_fine_products_t(Ctx* c, int launches, double* ms) {
 }
 }  // namespace
 
-int amg_time_fine_products(Ctx* c, int launches, double* ms) {
+int amg_time_level_products(Ctx* c, int level, int launches, double* ms) {
   if (!c->amg.built || !c->amg.lam_set)
     return set_err(c, ONSAS_ERR_STATE, "no multigrid hierarchy: run a PCG solve with ONSAS_PRECOND_AMG first");
   if (launches < 2 || launches > 10000) return set_err(c, ONSAS_ERR_ARG, "launches must be in [2, 10000]");
-  return c->precision == ONSAS_PRECISION_FP64 ? amg_time_fine_products_t<double>(c, launches, ms)
-                                              : amg_time_fine_products_t<float>(c, launches, ms);
+  const int nl = (int)c->amg.lev.size();
+  if (level < 0 || level + 1 >= nl)
+    return set_err(c, ONSAS_ERR_ARG, "level must be a smoothed level (0 .. levels - 2)");
+  if (level > 0 && c->amg.lev[level].nslices6 == 0)
+    return set_err(c, ONSAS_ERR_ARG, "this level has no sliced-ELL operator");
+  return c->precision == ONSAS_PRECISION_FP64
+             ? amg_time_level_products_t<double>(c, level, launches, ms)
+             : amg_time_level_products_t<float>(c, level, launches, ms);
 }
 
 int amg_nlevels(Ctx* c) { return c->amg.built && c->amg.lam_set ? (int)c->amg.lev.size() : 0; }
@@ -1747,7 +1791,7 @@ int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double
   if (vals) {
     const size_t m = (size_t)L.bs * 6 * L.nnz_p;
     if (c->precision == ONSAS_PRECISION_FP64) {
-      std::vector<float> h;
+      std::vector<double> h;
       ONSAS_TRY(copy_back(c, h, L.d_P, m));
       std::copy(h.begin(), h.end(), vals);
     } else {
@@ -1770,7 +1814,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
                     (void*)L.d_s6_row, L.d_A6s})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part, A.d_Ks,
+  for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part,
                   (void*)A.d_rzs})
     if (p) (void)hipFree(p);
   A = Amg{};

@@ -27,6 +27,7 @@ namespace {
 
 std::string g_create_err;
 
+
 Ctx* C(onsas_ctx* ctx) { return ctx ? &ctx->impl : nullptr; }
 
 int check_ready(Ctx* c) {
@@ -191,8 +192,7 @@ int assemble_now(Ctx* c) {
   ONSAS_TRY(onsas::launch_assemble(c));
   if (!c->gather) ONSAS_TRY(onsas::launch_pack_sell(c));
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  c->amg.stale = true;     // its Galerkin products belong to the previous tangent
-  c->amg.ks_stale = true;  // ... and so does the smoother copy of K
+  c->amg.stale = true;  // its Galerkin products belong to the previous tangent
   return ONSAS_OK;
 }
 
@@ -609,7 +609,15 @@ int onsas_time_cycle_products(onsas_ctx* ctx, int32_t launches, double* ms_per_l
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (!ms_per_launch) return set_err(c, ONSAS_ERR_ARG, "NULL output");
-  return onsas::amg_time_fine_products(c, launches, ms_per_launch);
+  return onsas::amg_time_level_products(c, 0, launches, ms_per_launch);
+}
+
+int onsas_time_level_products(onsas_ctx* ctx, int32_t level, int32_t launches,
+                              double* ms_per_launch) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!ms_per_launch) return set_err(c, ONSAS_ERR_ARG, "NULL output");
+  return onsas::amg_time_level_products(c, level, launches, ms_per_launch);
 }
 
 int onsas_amg_levels(onsas_ctx* ctx, int32_t* nlevels) {

@@ -124,14 +124,16 @@ struct AmgLevel {
   // rows ordered by decreasing length within windows, slices of R = rpw6 of
   // them, one wave each; lane l = R ph + r serves row s6_row[R s + r] (-1:
   // none), slot (64 / R) m + ph of round m.  Round q of the level (s6_ptr[s] <= q < s6_ptr[s+1])
-  // holds lane l's block column at s6_col[64 q + l] and its 36 values in 9
-  // chunks of 4 at A6s[((9 q + chunk) 64 + l) 4 + j], rounded to BF16 (FP32 in
-  // FP64 mode): every product of the cycle on this level reads that copy.
+  // holds lane l's block column at s6_col[64 q + l] and its 36 values at
+  // A6s[2304 q + ...] in lane-interleaved 16-byte chunks (Sell6Layout in
+  // amg.hip), BF16 in mixed mode, FP64 in FP64 mode: every product of the
+  // cycle on this level reads that copy.
   // s6_src is the source block in A (-1: padding, zero values).
   int64_t nslices6 = 0, nrounds6 = 0;
   int rpw6 = 8;                       // rows per slice: 8, 4 or 2
+  int ur6 = 4;                        // rounds whose loads a lane issues together
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
-  void* d_A6s = nullptr;              // the sliced-ELL values (BF16 mixed, FP32 FP64)
+  void* d_A6s = nullptr;              // the sliced-ELL values (BF16 mixed, FP64 FP64)
   double lam = 0;                     // lambda_max(D^-1 A), power iteration
   void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)
   // cycle vectors (KT, bs n): right-hand side, iterate, residual, Chebyshev steps
@@ -162,8 +164,6 @@ struct Amg {
   int64_t gen = 0;         // numerics generation: captured launches bake in its
                            // smoothing coefficients
   bool stale = true;       // the coarse levels belong to an earlier tangent
-  bool ks_stale = true;    // the smoother copy of the fine tangent is older than K
-  void* d_Ks = nullptr;    // that copy (FP64 mode): the sliced-ELL values in FP32
   bool step_stale = true;  // ... to an earlier load step
   int fine_degree = 1;     // Chebyshev degree of the finest level's smoothing (1 or 2)
   std::vector<AmgLevel> lev;
@@ -366,14 +366,13 @@ int amg_numeric(Ctx* c);                                        // Galerkin, inv
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
                   hipEvent_t* ev = nullptr, double* prz = nullptr);
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
-int amg_prepare(Ctx* c);  // refresh the smoother copy of the tangent (outside graphs)
 void amg_free(Ctx* c);
 int amg_nlevels(Ctx* c);  // hierarchy inspection (0 before it is built)
 int amg_export_info(Ctx* c, int l, int64_t* n, int32_t* bs, int64_t* nnzb, int64_t* nnz_p,
                     double* lam);
 int amg_export_operator(Ctx* c, int l, int32_t* row_ptr, int32_t* col, double* vals);
 int amg_export_prolongator(Ctx* c, int l, int32_t* p_ptr, int32_t* p_col, double* vals);
-int amg_time_fine_products(Ctx* c, int launches, double* ms);  // batch timing
+int amg_time_level_products(Ctx* c, int level, int launches, double* ms);  // batch timing
 
 // ---- solver_kernels.hip ----
 int launch_residual(Ctx* c);                                    // d_res, partial R2/F2

@@ -979,7 +979,6 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
       ONSAS_TRY(amg_numeric(c));
       c->amg.step_stale = false;
     }
-    ONSAS_TRY(amg_prepare(c));
   }
   if (x0) {
     // warm start: t = K x0 (into q, free until the first iteration), the scale

@@ -34,11 +34,12 @@ EXPORTED = (
     "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_snapshot_U",
     "onsas_read_snapshot", "onsas_set_U", "onsas_get_fext",
     "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
-    "onsas_tangent_matvec", "onsas_element_forces", "onsas_get_stress_strain",
+    "onsas_tangent_matvec", "onsas_element_forces",
+    "onsas_get_stress_strain",
     "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
     "onsas_get_timings", "onsas_reset_timings", "onsas_inject_error", "onsas_amg_levels",
     "onsas_amg_level_info", "onsas_amg_get_operator", "onsas_amg_get_prolongator",
-    "onsas_time_cycle_products",
+    "onsas_time_cycle_products", "onsas_time_level_products",
 )
 
 
@@ -134,6 +135,8 @@ def load():
         "onsas_amg_get_operator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
         "onsas_amg_get_prolongator": ([_vp, C.c_int32, _vp, _vp, _vp], C.c_int),
         "onsas_time_cycle_products": ([_vp, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "onsas_time_level_products": ([_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
+                                      C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -349,6 +352,15 @@ class Context:
         `launches` back-to-back launches between two events on the stream."""
         ms = C.c_double(0)
         self._check(self.L.onsas_time_cycle_products(self.h, int(launches), C.byref(ms)))
+        return ms.value
+
+    def time_level_products(self, level, launches=50):
+        """ms per launch of a multigrid level's cycle products (level 0: the two
+        fine-level ones; a sliced-ELL level >= 1: its four, in cycle order),
+        `launches` back-to-back launches between two events on the stream."""
+        ms = C.c_double(0)
+        self._check(self.L.onsas_time_level_products(self.h, int(level), int(launches),
+                                                     C.byref(ms)))
         return ms.value
 
     def tangent_bsr(self):

@@ -6,10 +6,13 @@ Mesh(::MshFile, ::StructuralEntity) of
 src/StructuralModel/StructuralEntities.jl:76-156.  The reference reads the file
 with MshReader.jl (not vendored; its MshFileReader returns node coordinates,
 element connectivities, the physical names indexed by physical tag and each
-element's physical tag).  `read_msh` restates that reader for the ASCII MSH
-2.2 format: $PhysicalNames, $Nodes, $Elements (element types 15 point,
-1 line, 2 triangle, 4 tetrahedron; the first tag is the physical tag).
-Node ids become 0-based in file order.
+element's physical tag).  `read_msh` restates that reader from the published
+Gmsh file format for ASCII files of version 2.2 and 4.1 (4.1 is what
+`gmsh.write` produces by default, e.g. examples/uniaxial_extension/
+uniaxial_mesh.jl:89-90): $PhysicalNames, $Nodes, $Elements, and in 4.1 the
+$Entities section that carries the physical tags.  Element types 15 point,
+1 line, 2 triangle, 4 tetrahedron.  Nodes are numbered 0-based in ascending
+node-tag order (the order gmsh writes them in).
 """
 from __future__ import annotations
 
@@ -22,49 +25,127 @@ _NODES_PER_TYPE = {15: 1, 1: 2, 2: 3, 4: 4}
 
 
 def read_msh(source):
-    """MSH 2.2 ASCII reader -> (coords (nn, 3), connectivity [int array per
+    """ASCII MSH 2.2 or 4.1 reader -> (coords (nn, 3), connectivity [int array per
     element, 0-based], physical_names [name of tag 1, 2, ...], physical_index
-    [1-based tag per element])."""
+    [1-based tag per element, 0 when the element has none]).
+
+    MSH 2.2 gives an element's physical tag as its first tag.  MSH 4.1 gives it
+    through the element's entity ($Entities): an element of an entity with k
+    physical tags is listed k times, once per tag, as the same mesh saved in
+    MSH 2.2 lists it; an entity without one gives tag 0."""
     text = source if "\n" in source else open(source, encoding="utf-8").read()
     lines = iter(text.splitlines())
-    coords, ids = None, None
-    conn, phys = [], []
+    version = None
     names = {}
     for line in lines:
         tag = line.strip()
         if tag == "$MeshFormat":
             version, ftype, _ = next(lines).split()
-            if not version.startswith("2") or ftype != "0":
-                raise ValueError(f"only ASCII MSH 2.x is read (got version {version}, type {ftype})")
-        elif tag == "$PhysicalNames":
-            for _ in range(int(next(lines))):
-                dim, ptag, name = next(lines).split(maxsplit=2)
-                names[int(ptag)] = name.strip().strip('"')
-        elif tag == "$Nodes":
-            n = int(next(lines))
-            ids = np.empty(n, dtype=np.int64)
-            coords = np.empty((n, 3))
-            for k in range(n):
-                f = next(lines).split()
-                ids[k] = int(f[0])
-                coords[k] = [float(f[1]), float(f[2]), float(f[3])]
-        elif tag == "$Elements":
-            pos = {int(i): k for k, i in enumerate(ids)}
-            for _ in range(int(next(lines))):
-                f = [int(v) for v in next(lines).split()]
-                etype, ntags = f[1], f[2]
-                if etype not in _NODES_PER_TYPE:
-                    raise ValueError(f"element type {etype} is not supported")
-                nodes = f[3 + ntags:]
-                if len(nodes) != _NODES_PER_TYPE[etype]:
-                    raise ValueError(f"element {f[0]}: {len(nodes)} nodes for type {etype}")
-                conn.append(np.array([pos[v] for v in nodes], dtype=np.int64))
-                phys.append(f[3] if ntags > 0 else 0)
+            if ftype != "0":
+                raise ValueError(f"only ASCII MSH files are read (file type {ftype})")
+            if version.startswith("2"):
+                return _read_msh2(lines, names)
+            if version.startswith("4.1"):
+                return _read_msh41(lines, names)
+            raise ValueError(f"MSH version {version} is not read (2.x and 4.1 are)")
+        if tag == "$PhysicalNames":
+            _read_names(lines, names)
+    return _read_msh2(iter(text.splitlines()), names)  # no $MeshFormat: 2.x layout
+
+
+def _read_names(lines, names):
+    for _ in range(int(next(lines))):
+        dim, ptag, name = next(lines).split(maxsplit=2)
+        names[int(ptag)] = name.strip().strip('"')
+
+
+def _finish(tags, coords, conn_tags, phys, names):
     if coords is None:
         raise ValueError("no $Nodes section")
+    order = np.argsort(tags, kind="stable")
+    tags, coords = tags[order], coords[order]
+    pos = {int(t): k for k, t in enumerate(tags)}
+    conn = [np.array([pos[v] for v in c], dtype=np.int64) for c in conn_tags]
     ntag = max(names) if names else 0
     physical_names = [names.get(t, "") for t in range(1, ntag + 1)]
     return coords, conn, physical_names, np.array(phys, dtype=np.int64)
+
+
+def _element_nodes(etype, nodes, etag):
+    if etype not in _NODES_PER_TYPE:
+        raise ValueError(f"element type {etype} is not supported")
+    if len(nodes) != _NODES_PER_TYPE[etype]:
+        raise ValueError(f"element {etag}: {len(nodes)} nodes for type {etype}")
+    return nodes
+
+
+def _read_msh2(lines, names):
+    tags = coords = None
+    conn, phys = [], []
+    for line in lines:
+        tag = line.strip()
+        if tag == "$PhysicalNames":
+            _read_names(lines, names)
+        elif tag == "$Nodes":
+            n = int(next(lines))
+            tags = np.empty(n, dtype=np.int64)
+            coords = np.empty((n, 3))
+            for k in range(n):
+                f = next(lines).split()
+                tags[k] = int(f[0])
+                coords[k] = [float(f[1]), float(f[2]), float(f[3])]
+        elif tag == "$Elements":
+            for _ in range(int(next(lines))):
+                f = [int(v) for v in next(lines).split()]
+                etype, ntags = f[1], f[2]
+                conn.append(_element_nodes(etype, f[3 + ntags:], f[0]))
+                phys.append(f[3] if ntags > 0 else 0)
+    return _finish(tags, coords, conn, phys, names)
+
+
+def _read_msh41(lines, names):
+    ent_phys = {}  # (dim, entity tag) -> physical tags
+    tags = coords = None
+    conn, phys = [], []
+    for line in lines:
+        tag = line.strip()
+        if tag == "$PhysicalNames":
+            _read_names(lines, names)
+        elif tag == "$Entities":
+            counts = [int(v) for v in next(lines).split()]
+            for dim, cnt in enumerate(counts):
+                for _ in range(cnt):
+                    f = next(lines).split()
+                    # points: tag x y z nphys ...; others: tag 6 bounds nphys ...
+                    k = 4 if dim == 0 else 7
+                    nphys = int(f[k])
+                    ent_phys[(dim, int(f[0]))] = [int(v) for v in f[k + 1:k + 1 + nphys]]
+        elif tag == "$Nodes":
+            nblocks, nnodes, _, _ = (int(v) for v in next(lines).split())
+            tags = np.empty(nnodes, dtype=np.int64)
+            coords = np.empty((nnodes, 3))
+            k = 0
+            for _ in range(nblocks):
+                dim, etag, param, nb = (int(v) for v in next(lines).split())
+                tags[k:k + nb] = [int(next(lines)) for _ in range(nb)]
+                for q in range(nb):  # x y z (and u [v [w]] when parametric)
+                    f = next(lines).split()
+                    coords[k + q] = [float(f[0]), float(f[1]), float(f[2])]
+                k += nb
+            if k != nnodes:
+                raise ValueError(f"$Nodes lists {k} nodes, its header {nnodes}")
+        elif tag == "$Elements":
+            nblocks, _, _, _ = (int(v) for v in next(lines).split())
+            for _ in range(nblocks):
+                dim, etag, etype, nb = (int(v) for v in next(lines).split())
+                ptags = ent_phys.get((dim, etag)) or [0]
+                for _ in range(nb):
+                    f = [int(v) for v in next(lines).split()]
+                    nodes = _element_nodes(etype, f[1:], f[0])
+                    for pt in ptags:
+                        conn.append(nodes)
+                        phys.append(pt)
+    return _finish(tags, coords, conn, phys, names)
 
 
 def _getlabels(physical_names):

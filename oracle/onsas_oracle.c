@@ -479,6 +479,8 @@ void or_csr_free(or_csr* A) {
 }
 
 void or_csr_matvec(const or_csr* A, const double* x, double* y) {
+    /* rows are independent: the OpenMP build (CPU baseline only) splits them */
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < A->n; ++i) {
         double s = 0;
         for (int64_t k = A->rowptr[i]; k < A->rowptr[i + 1]; ++k) s += A->val[k] * x[A->col[k]];
@@ -488,7 +490,11 @@ void or_csr_matvec(const or_csr* A, const double* x, double* y) {
 
 /* Build a CSR matrix from COO triplets, summing duplicates in triplet order:
  * the result of `K[I[k], J[k]] += V[k]` for k = 1..end (Assemblers.jl:84-88).
- * rmap/cmap renumber (or drop, when < 0) the rows and columns. */
+ * map renumbers (or drops, when < 0) the rows and columns.  Triplets are
+ * bucketed by row (stable), then each row's distinct columns get a slot the
+ * first time they appear (a marker per column), every value is added to its
+ * slot in triplet order starting from a structural 0.0, and the row's slots are
+ * sorted by column (insertion sort of the distinct columns, a few dozen). */
 static int coo_to_csr(int64_t nt, const int32_t* I, const int32_t* J, const double* V,
                       int64_t n, const int64_t* map, or_csr* A) {
     int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
@@ -503,7 +509,8 @@ static int coo_to_csr(int64_t nt, const int32_t* I, const int32_t* J, const doub
     int32_t* tc = (int32_t*)malloc((size_t)(m ? m : 1) * sizeof(int32_t));
     double* tv = (double*)malloc((size_t)(m ? m : 1) * sizeof(double));
     int64_t* pos = (int64_t*)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
-    if (!tc || !tv || !pos) return OR_ERR_ALLOC;
+    int64_t* mark = (int64_t*)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+    if (!tc || !tv || !pos || !mark) return OR_ERR_ALLOC;
     memcpy(pos, cnt, (size_t)n * sizeof(int64_t));
     for (int64_t k = 0; k < nt; ++k) { /* stable bucket by row */
         int64_t r = map ? map[I[k]] : I[k];
@@ -513,34 +520,52 @@ static int coo_to_csr(int64_t nt, const int32_t* I, const int32_t* J, const doub
         tv[pos[r]] = V[k];
         pos[r]++;
     }
-    /* per row: stable insertion sort by column, then sum runs in order */
+    for (int64_t i = 0; i < n; ++i) mark[i] = -1;
     A->n = n;
     A->rowptr = (int64_t*)malloc(((size_t)n + 1) * sizeof(int64_t));
     A->col = (int32_t*)malloc((size_t)(m ? m : 1) * sizeof(int32_t));
     A->val = (double*)malloc((size_t)(m ? m : 1) * sizeof(double));
-    if (!A->rowptr || !A->col || !A->val) return OR_ERR_ALLOC;
+    /* (column, slot) pairs of one row, for the sort */
+    int64_t cap = 64;
+    int32_t* pair = (int32_t*)malloc((size_t)cap * 2 * sizeof(int32_t));
+    double* rowv = (double*)malloc((size_t)cap * sizeof(double));
+    if (!A->rowptr || !A->col || !A->val || !pair || !rowv) return OR_ERR_ALLOC;
     int64_t out = 0;
     A->rowptr[0] = 0;
     for (int64_t r = 0; r < n; ++r) {
-        int64_t b = cnt[r], e = cnt[r + 1];
-        for (int64_t k = b + 1; k < e; ++k) {
+        int64_t b = cnt[r], e = cnt[r + 1], nd = 0;
+        for (int64_t k = b; k < e; ++k) {
             int32_t c = tc[k];
-            double v = tv[k];
-            int64_t q = k - 1;
-            while (q >= b && tc[q] > c) {
-                tc[q + 1] = tc[q];
-                tv[q + 1] = tv[q];
-                --q;
+            if (mark[c] < 0) { /* first appearance in this row: a new slot */
+                if (nd == cap) {
+                    cap *= 2;
+                    pair = (int32_t*)realloc(pair, (size_t)cap * 2 * sizeof(int32_t));
+                    rowv = (double*)realloc(rowv, (size_t)cap * sizeof(double));
+                    if (!pair || !rowv) return OR_ERR_ALLOC;
+                }
+                mark[c] = nd;
+                pair[2 * nd] = c;
+                pair[2 * nd + 1] = (int32_t)nd;
+                rowv[nd] = 0.0; /* structural zero then += in triplet order */
+                ++nd;
             }
-            tc[q + 1] = c;
-            tv[q + 1] = v;
+            rowv[mark[c]] += tv[k];
         }
-        for (int64_t k = b; k < e;) {
-            int32_t c = tc[k];
-            double s = 0.0; /* structural zero then += in order */
-            while (k < e && tc[k] == c) s += tv[k++];
-            A->col[out] = c;
-            A->val[out] = s;
+        for (int64_t q = 1; q < nd; ++q) { /* distinct columns, ascending */
+            int32_t c = pair[2 * q], sl = pair[2 * q + 1];
+            int64_t w = q - 1;
+            while (w >= 0 && pair[2 * w] > c) {
+                pair[2 * w + 2] = pair[2 * w];
+                pair[2 * w + 3] = pair[2 * w + 1];
+                --w;
+            }
+            pair[2 * w + 2] = c;
+            pair[2 * w + 3] = sl;
+        }
+        for (int64_t q = 0; q < nd; ++q) {
+            A->col[out] = pair[2 * q];
+            A->val[out] = rowv[pair[2 * q + 1]];
+            mark[pair[2 * q]] = -1;
             ++out;
         }
         A->rowptr[r + 1] = out;
@@ -550,6 +575,9 @@ static int coo_to_csr(int64_t nt, const int32_t* I, const int32_t* J, const doub
     free(tc);
     free(tv);
     free(pos);
+    free(mark);
+    free(pair);
+    free(rowv);
     return OR_OK;
 }
 
@@ -569,9 +597,16 @@ int or_assemble(const or_model* m, const double* U, double* fint, or_csr* Kfull,
     memset(fint, 0, (size_t)n * sizeof(double));
     int64_t t = 0;
     int st = OR_OK;
-    /* tetrahedra (Tetrahedrons.jl via Entities.jl:211-218) */
-    for (int64_t e = 0; e < m->ntet && st == OR_OK; ++e) {
-        double X[12], ue[12], fe[12], Ke[144], se[9], ee[9];
+    /* tetrahedra (Tetrahedrons.jl via Entities.jl:211-218): element e's 144
+     * triplets sit at 144 e, column-major (Assemblers.jl:59-66), and its forces
+     * are summed into fint in element order after the loop, so the OpenMP
+     * build (CPU baseline only) gives the same arrays as the serial one */
+    double* FE = (double*)malloc((size_t)(m->ntet ? m->ntet : 1) * 12 * sizeof(double));
+    if (!FE) return OR_ERR_ALLOC;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < m->ntet; ++e) {
+        double X[12], ue[12], Ke[144], se[9], ee[9];
+        double* fe = &FE[e * 12];
         int32_t dofs[12];
         for (int a = 0; a < 4; ++a) {
             int32_t nd = m->tet[e * 4 + a];
@@ -582,22 +617,35 @@ int or_assemble(const or_model* m, const double* U, double* fint, or_csr* Kfull,
             }
         }
         double G = m->tet_G ? m->tet_G[e] : m->tet_p1;
+        int ste;
         if (m->tet_mat == OR_MAT_LINEAR)
-            st = or_tet_linear(m->tet_p0, m->tet_p1, X, ue, fe, Ke, se, ee);
+            ste = or_tet_linear(m->tet_p0, m->tet_p1, X, ue, fe, Ke, se, ee);
         else
-            st = or_tet_hyperelastic(m->tet_mat, m->tet_p0, G, X, ue, fe, Ke, se, ee);
-        if (st != OR_OK) break;
-        for (int r = 0; r < 12; ++r) fint[dofs[r]] += fe[r]; /* StructuralAnalyses.jl:105-107 */
-        for (int c = 0; c < 12; ++c)                         /* column-major (Assemblers.jl:59-66) */
+            ste = or_tet_hyperelastic(m->tet_mat, m->tet_p0, G, X, ue, fe, Ke, se, ee);
+        if (ste != OR_OK) {
+#pragma omp atomic write
+            st = ste;
+            continue;
+        }
+        int64_t tt = 144 * e;
+        for (int c = 0; c < 12; ++c)
             for (int r = 0; r < 12; ++r) {
-                I[t] = dofs[r];
-                J[t] = dofs[c];
-                V[t] = Ke[r * 12 + c];
-                ++t;
+                I[tt] = dofs[r];
+                J[tt] = dofs[c];
+                V[tt] = Ke[r * 12 + c];
+                ++tt;
             }
         if (stress) memcpy(&stress[e * 9], se, sizeof se);
         if (strain) memcpy(&strain[e * 9], ee, sizeof ee);
     }
+    if (st == OR_OK) {
+        for (int64_t e = 0; e < m->ntet; ++e) /* StructuralAnalyses.jl:105-107 */
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d)
+                    fint[3 * (int64_t)m->tet[e * 4 + a] + d] += FE[e * 12 + a * 3 + d];
+        t = 144 * m->ntet;
+    }
+    free(FE);
     for (int64_t e = 0; e < m->ntruss && st == OR_OK; ++e) {
         double X[6], ue[6], fe[6], Ke[36], s11, e11;
         int32_t dofs[6];
@@ -652,10 +700,14 @@ int or_assemble(const or_model* m, const double* U, double* fint, or_csr* Kfull,
 /* ------------------------------------------------------------------------ */
 static double nrm2(const double* x, int64_t n) {
     double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
     for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
     return sqrt(s);
 }
 
+/* The OpenMP pragmas take effect only in the CPU-baseline build of this file
+ * (-fopenmp); there the reductions' summation order depends on the thread
+ * count, so only the serial build is the checker. */
 int64_t or_cg(const or_csr* A, const double* b, double* x, double abstol, double reltol,
               int64_t maxiter) {
     int64_t n = A->n;
@@ -671,11 +723,14 @@ int64_t or_cg(const or_csr* A, const double* b, double* x, double abstol, double
     int64_t it = 0;
     while (!(residual <= tol) && it < maxiter) {
         double beta = residual * residual / (prev * prev);
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) u[i] = r[i] + beta * u[i];
         or_csr_matvec(A, u, c);
         double uc = 0;
+#pragma omp parallel for reduction(+ : uc) schedule(static)
         for (int64_t i = 0; i < n; ++i) uc += u[i] * c[i];
         double alpha = residual * residual / uc;
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
             x[i] += alpha * u[i];
             r[i] -= alpha * c[i];

@@ -17,6 +17,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libonsas_oracle.so")
+# the same source built with OpenMP: the all-cores CPU baseline only (its
+# reductions' order depends on the thread count, so it is not the checker)
+_LIB_OMP_PATH = os.path.join(_HERE, "_build", "libonsas_oracle_omp.so")
 
 MAT_SVK, MAT_NEOHOOKEAN, MAT_LINEAR = 1, 2, 3
 STRAIN_ROTATED, STRAIN_GREEN = 0, 1
@@ -48,7 +51,7 @@ class StepRecord(C.Structure):
                 ("cg_iters", C.c_int64)]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
@@ -56,12 +59,13 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(omp=False):
+    """The serial oracle (the checker), or with omp=True its OpenMP build."""
+    if omp not in _libs:
+        path = _LIB_OMP_PATH if omp else _LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         L.or_svk_stress.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_svk_energy.argtypes = [C.c_double, C.c_double, _dp]
         L.or_svk_energy.restype = C.c_double
@@ -86,8 +90,8 @@ def lib():
         L.or_newton_static.argtypes = [C.POINTER(_Model), C.c_int, _dp, C.c_double, C.c_double,
                                        C.c_int, C.c_double, _dp, C.c_void_p,
                                        C.POINTER(StepRecord)]
-        _lib = L
-    return _lib
+        _libs[omp] = L
+    return _libs[omp]
 
 
 class OracleError(RuntimeError):
@@ -190,7 +194,8 @@ class Model:
 
     def __init__(self, X, tets=None, tet_mat=MAT_SVK, tet_p=(0.0, 0.0), tet_G=None,
                  trusses=None, truss_strain=STRAIN_ROTATED, truss_E=0.0, truss_A=None,
-                 fixed_dofs=()):
+                 fixed_dofs=(), omp=False):
+        self.L = lib(omp)  # omp: the OpenMP build (CPU baseline timing only)
         self.X = _f64(X)
         self.nn = self.X.size // 3
         self.n = 3 * self.nn
@@ -231,7 +236,7 @@ class Model:
         ne = self.ntet + self.ntruss
         stress = np.zeros(9 * ne) if want_stress else None
         strain = np.zeros(9 * ne) if want_stress else None
-        st = lib().or_assemble(C.byref(self._m), U, fint,
+        st = self.L.or_assemble(C.byref(self._m), U, fint,
                                C.byref(Kf) if want_full else None,
                                C.byref(Kff) if want_ff else None,
                                stress.ctypes.data if want_stress else None,
@@ -245,7 +250,7 @@ class Model:
             rp = np.ctypeslib.as_array(M.rowptr, shape=(n + 1,)).copy()
             col = np.ctypeslib.as_array(M.col, shape=(max(nnz, 1),))[:nnz].copy()
             val = np.ctypeslib.as_array(M.val, shape=(max(nnz, 1),))[:nnz].copy()
-            lib().or_csr_free(C.byref(M))
+            self.L.or_csr_free(C.byref(M))
             out[name] = sp.csr_matrix((val, col, rp), shape=(n, n))
         if want_stress:
             out["stress"] = stress.reshape(ne, 3, 3)
@@ -261,7 +266,7 @@ class Model:
         U = np.zeros(self.n) if U0 is None else _f64(U0, self.n).copy()
         Us = np.zeros((nsteps, self.n)) if keep_steps else None
         recs = (StepRecord * nsteps)()
-        st = lib().or_newton_static(C.byref(self._m), nsteps, F.reshape(-1), rel_U_tol,
+        st = self.L.or_newton_static(C.byref(self._m), nsteps, F.reshape(-1), rel_U_tol,
                                     rel_res_tol, max_iter, cg_reltol, U,
                                     Us.ctypes.data if keep_steps else None, recs)
         _check(st)
@@ -288,7 +293,7 @@ def coo_sum(I, J, V, n):
     return _csr_to_scipy(M)
 
 
-def cg(A, b, reltol=1.4901161193847656e-08, abstol=0.0, maxiter=None):
+def cg(A, b, reltol=1.4901161193847656e-08, abstol=0.0, maxiter=None, omp=False):
     """IterativeSolvers 0.9.4 cg! on a scipy CSR matrix (x0 = 0, no preconditioner)."""
     A = A.tocsr()
     n = A.shape[0]
@@ -298,5 +303,6 @@ def cg(A, b, reltol=1.4901161193847656e-08, abstol=0.0, maxiter=None):
     M = _CSR(n, val.size, rp.ctypes.data_as(C.POINTER(C.c_int64)),
              col.ctypes.data_as(C.POINTER(C.c_int32)), val.ctypes.data_as(C.POINTER(C.c_double)))
     x = np.zeros(n)
-    it = lib().or_cg(C.byref(M), _f64(b, n), x, abstol, reltol, n if maxiter is None else maxiter)
+    it = lib(omp).or_cg(C.byref(M), _f64(b, n), x, abstol, reltol,
+                        n if maxiter is None else maxiter)
     return x, it

@@ -186,3 +186,18 @@ def test_library_version_and_no_device_error():
     with pytest.raises(_capi.OnsasError) as e:
         _capi.Context()
     assert e.value.code == _capi.ERR_HIP
+
+
+def test_integration_c_example_compiles(tmp_path):
+    """The C usage example of INTEGRATION.md compiles against include/onsas_hip.h
+    (warnings are errors), so its calls match the declared signatures."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    doc = open(os.path.join(root, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```c\n(.*?)```", doc, flags=re.S)
+    assert len(blocks) == 1
+    src = tmp_path / "example.c"
+    src.write_text(blocks[0])
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(root, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -111,3 +111,136 @@ def test_vtk_collection(tmp_path):
     assert [e.get("file") for e in ds] == ["run_timestep_1.vtu", "run_timestep_2.vtu"]
     d = read_vtu(str(tmp_path / "run_timestep_1.vtu"))
     np.testing.assert_array_equal(d["point"]["Displacement"], sol.U[0].reshape(-1, 3))
+
+
+FIXTURE41 = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                       "cube_manufactured_v41.msh")
+# The same mesh as the MSH 4.1 fixture, written by hand in MSH 2.2 (elements
+# carry their physical tag first, their elementary entity second).
+CUBE22 = """$MeshFormat
+2.2 0 8
+$EndMeshFormat
+$PhysicalNames
+6
+0 6 "_node_origin"
+2 1 "_triangle_load"
+2 2 "_triangle_fixed-ux"
+2 3 "_triangle_fixed-uy"
+2 4 "_triangle_fixed-uz"
+3 5 "svk_tetrahedron_"
+$EndPhysicalNames
+$Nodes
+8
+1 0 0 0
+2 0 0 1
+3 0 1 1
+4 0 1 0
+5 1 0 0
+6 1 0 1
+7 1 1 1
+8 1 1 0
+$EndNodes
+$Elements
+15
+1 15 2 6 1 1
+2 2 2 1 1 5 8 6
+3 2 2 1 1 6 8 7
+4 2 2 2 2 4 1 2
+5 2 2 2 2 4 2 3
+6 2 2 3 3 6 2 1
+7 2 2 3 3 6 1 5
+8 2 2 4 4 1 4 5
+9 2 2 4 4 4 8 5
+10 4 2 5 1 1 4 2 6
+11 4 2 5 1 6 2 3 4
+12 4 2 5 1 4 3 6 7
+13 4 2 5 1 4 1 5 6
+14 4 2 5 1 4 6 5 8
+15 4 2 5 1 4 7 6 8
+$EndElements
+"""
+
+
+def test_read_msh41_fixture():
+    """A hand-written MSH 4.1 file (the manufactured cube of
+    examples/uniaxial_extension/uniaxial_extension.jl:45-73 with the physical
+    names of its Gmsh case): nodes by entity block, physical tags through
+    $Entities."""
+    coords, conn, names, phys = read_msh(FIXTURE41)
+    np.testing.assert_array_equal(coords, [[0, 0, 0], [0, 0, 1], [0, 1, 1], [0, 1, 0],
+                                           [1, 0, 0], [1, 0, 1], [1, 1, 1], [1, 1, 0]])
+    assert names == ["_triangle_load", "_triangle_fixed-ux", "_triangle_fixed-uy",
+                     "_triangle_fixed-uz", "svk_tetrahedron_", "_node_origin"]
+    np.testing.assert_array_equal(phys, [6, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 5, 5, 5, 5])
+    assert [c.tolist() for c in conn[:3]] == [[0], [4, 7, 5], [5, 7, 6]]
+    np.testing.assert_array_equal(np.array(conn[9:]), [[0, 3, 1, 5], [5, 1, 2, 3], [3, 2, 5, 6],
+                                                       [3, 0, 4, 5], [3, 5, 4, 7], [3, 6, 5, 7]])
+
+
+def test_msh41_and_msh22_of_one_mesh_agree():
+    a, b = read_msh(FIXTURE41), read_msh(CUBE22)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert [c.tolist() for c in a[1]] == [c.tolist() for c in b[1]]
+    assert a[2] == b[2]
+    np.testing.assert_array_equal(a[3], b[3])
+
+
+def test_msh41_entity_blocks_and_tags():
+    """Node tags out of file order (numbered by ascending tag), a parametric node
+    block (x y z u v), an entity in two physical groups (its elements listed once
+    per group, as MSH 2.2 lists them) and one in none (tag 0)."""
+    src = """$MeshFormat
+4.1 0 8
+$EndMeshFormat
+$PhysicalNames
+2
+2 1 "_triangle_a"
+2 2 "_triangle_b"
+$EndPhysicalNames
+$Entities
+0 0 2 0
+1 0 0 0 1 1 0 2 1 2 0
+2 0 0 0 1 1 0 0 0
+$EndEntities
+$Nodes
+2 4 1 4
+2 1 0 2
+4
+3
+1 1 0
+0 1 0
+2 1 1 2
+1
+2
+0 0 0 0.0 0.0
+1 0 0 1.0 0.0
+$EndNodes
+$Elements
+2 2 1 2
+2 1 2 1
+1 1 2 3
+2 2 2 1
+2 2 3 4
+$EndElements
+"""
+    coords, conn, names, phys = read_msh(src)
+    np.testing.assert_array_equal(coords, [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]])
+    assert [c.tolist() for c in conn] == [[0, 1, 2], [0, 1, 2], [1, 2, 3]]
+    np.testing.assert_array_equal(phys, [1, 2, 0])
+    assert names == ["_triangle_a", "_triangle_b"]
+
+
+def test_msh41_fixture_builds_the_reference_example_structure():
+    msh = MshFile(FIXTURE41)
+    assert msh.entity_label(9) == "tetrahedron" and msh.material_label(9) == "svk"
+    gm = mesh_from_msh(msh, {"tetrahedron": Tetrahedra, "triangle": TriangularFaces})
+    np.testing.assert_array_equal(gm.face_set("load").conn, [[4, 7, 5], [5, 7, 6]])
+    assert gm.node_set("origin").ids.tolist() == [0]
+    assert len(gm.element_set("svk").conn) == 6
+
+
+def test_read_msh_rejects_other_versions():
+    with pytest.raises(ValueError):
+        read_msh("$MeshFormat\n4.0 0 8\n$EndMeshFormat\n")
+    with pytest.raises(ValueError):
+        read_msh("$MeshFormat\n4.1 1 8\n$EndMeshFormat\n")
