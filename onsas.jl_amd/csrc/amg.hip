@@ -37,6 +37,7 @@
 #include <hip/hip_ext.h>
 
 #include "engine.hpp"
+#include "host_setup.hpp"
 
 namespace onsas {
 
@@ -1035,97 +1036,6 @@ int sell6_rounds_in_flight(bool f64) {
   return f64 ? 2 : 4;
 }
 
-// greedy aggregation of a node graph (CSR, self included)
-int64_t aggregate(const std::vector<int32_t>& rp, const std::vector<int32_t>& cl, int64_t n,
-                  std::vector<int32_t>& agg) {
-  agg.assign(n, -1);
-  int32_t na = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    if (agg[i] >= 0) continue;
-    bool free = true;
-    for (int32_t k = rp[i]; k < rp[i + 1] && free; ++k) free = agg[cl[k]] < 0;
-    if (!free) continue;
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) agg[cl[k]] = na;
-    agg[i] = na++;
-  }
-  for (int64_t i = 0; i < n; ++i) {
-    if (agg[i] >= 0) continue;
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
-      if (agg[cl[k]] >= 0) {
-        agg[i] = agg[cl[k]];
-        break;
-      }
-  }
-  for (int64_t i = 0; i < n; ++i)
-    if (agg[i] < 0) agg[i] = na++;
-  return na;
-}
-
-// Tentative prolongator of one level: per aggregate, modified Gram-Schmidt
-// (twice) of the rows of its member nodes in the near-nullspace B (bs x 6 per
-// node); T gets Q, Bc (6 x 6 per aggregate) gets R.
-void tentative(const std::vector<int64_t>& aptr, const std::vector<int32_t>& anodes, int bs,
-               const std::vector<double>& B, std::vector<double>& T, std::vector<double>& Bc) {
-  const int64_t na = (int64_t)aptr.size() - 1;
-  std::vector<double> V, Q, col;
-  for (int64_t a = 0; a < na; ++a) {
-    const int64_t m = aptr[a + 1] - aptr[a];
-    const int64_t rows = m * bs;
-    V.assign(rows * 6, 0.0);
-    Q.assign(rows * 6, 0.0);
-    for (int64_t k = 0; k < m; ++k)
-      for (int r = 0; r < bs; ++r)
-        for (int q = 0; q < 6; ++q) V[(k * bs + r) * 6 + q] = B[anodes[aptr[a] + k] * bs * 6 + r * 6 + q];
-    double R[36] = {0};
-    for (int q = 0; q < 6; ++q) {
-      col.assign(rows, 0.0);
-      double n0 = 0;
-      for (int64_t t = 0; t < rows; ++t) {
-        col[t] = V[t * 6 + q];
-        n0 += col[t] * col[t];
-      }
-      n0 = std::sqrt(n0);
-      for (int pass = 0; pass < 2; ++pass)
-        for (int j = 0; j < q; ++j) {
-          double d = 0;
-          for (int64_t t = 0; t < rows; ++t) d += Q[t * 6 + j] * col[t];
-          R[j * 6 + q] += d;
-          for (int64_t t = 0; t < rows; ++t) col[t] -= d * Q[t * 6 + j];
-        }
-      double nv = 0;
-      for (int64_t t = 0; t < rows; ++t) nv += col[t] * col[t];
-      nv = std::sqrt(nv);
-      if (nv > 1e-10 * n0 && nv > 0) {
-        R[q * 6 + q] = nv;
-        for (int64_t t = 0; t < rows; ++t) Q[t * 6 + q] = col[t] / nv;
-      }
-    }
-    for (int64_t k = 0; k < m; ++k)
-      for (int r = 0; r < bs; ++r)
-        for (int q = 0; q < 6; ++q) T[anodes[aptr[a] + k] * bs * 6 + r * 6 + q] = Q[(k * bs + r) * 6 + q];
-    for (int q = 0; q < 36; ++q) Bc[a * 36 + q] = R[q];
-  }
-}
-
-struct Trip {  // (column key, two list entries)
-  int32_t key, a, b;
-};
-
-// Group the entries of one row by key (stable: generation order within a key):
-// appends the distinct keys to cols and each group's entries to la/lb, closing
-// each group in ptr.
-void group_row(std::vector<Trip>& trips, std::vector<int32_t>& cols, std::vector<int64_t>& ptr,
-               std::vector<int32_t>& la, std::vector<int32_t>* lb) {
-  std::stable_sort(trips.begin(), trips.end(), [](const Trip& x, const Trip& y) { return x.key < y.key; });
-  for (size_t t = 0; t < trips.size(); ++t) {
-    if (t > 0 && trips[t].key != trips[t - 1].key) ptr.push_back((int64_t)la.size());
-    if (t == 0 || trips[t].key != trips[t - 1].key) cols.push_back(trips[t].key);
-    la.push_back(trips[t].a);
-    if (lb) lb->push_back(trips[t].b);
-  }
-  if (!trips.empty()) ptr.push_back((int64_t)la.size());
-}
-
 }  // namespace
 
 // Build the hierarchy's symbolic part (host) and allocate its device arrays.
@@ -1149,203 +1059,98 @@ int amg_build(Ctx* c) {
   L0.n = nn;
   L0.nnzb = (int64_t)c->col.size();
   A.lev.push_back(L0);
-  // level-0 near-nullspace: translations now, rotations in the first pass below
+  // level-0 near-nullspace: translations and rotations
   std::vector<double> Bcur(nn * 3 * 6, 0.0);
   for (int64_t i = 0; i < nn; ++i)
     for (int d = 0; d < 3; ++d) Bcur[i * 18 + d * 6 + d] = 1.0;
+  // rigid-body rotations about ONE point (the node centroid) for the whole
+  // mesh: each aggregate's R then holds its coordinates of the same global
+  // modes, which the next level's tentative prolongator needs (rotations
+  // about a per-aggregate point span the same local space, but their R
+  // would describe a different, non-rigid field in every aggregate)
+  double cx = 0, cy = 0, cz = 0;
+  for (int64_t i = 0; i < nn; ++i) {
+    cx += c->X[i];
+    cy += c->Y[i];
+    cz += c->Z[i];
+  }
+  cx /= (double)nn;
+  cy /= (double)nn;
+  cz /= (double)nn;
+  for (int64_t i = 0; i < nn; ++i) {
+    const double x = c->X[i] - cx, y = c->Y[i] - cy, z = c->Z[i] - cz;
+    double* b = &Bcur[i * 18];
+    // rotations about x, y, z: columns 3, 4, 5 (rows: ux, uy, uz)
+    b[0 * 6 + 3] = 0;
+    b[1 * 6 + 3] = -z;
+    b[2 * 6 + 3] = y;
+    b[0 * 6 + 4] = z;
+    b[1 * 6 + 4] = 0;
+    b[2 * 6 + 4] = -x;
+    b[0 * 6 + 5] = -y;
+    b[1 * 6 + 5] = x;
+    b[2 * 6 + 5] = 0;
+    for (int d = 0; d < 3; ++d)
+      if ((c->fixmask[i] >> d) & 1u)
+        for (int q = 0; q < 6; ++q) b[d * 6 + q] = 0;
+  }
   std::vector<int32_t> rp(c->row_ptr), cl(c->col);
   int bs = 3;
   int64_t n = nn;
-  std::vector<Trip> trips;
-  for (int lev = 0;; ++lev) {
-    std::vector<int32_t> agg;
-    const int64_t na = aggregate(rp, cl, n, agg);
-    std::vector<int64_t> aptr(na + 1, 0);
-    for (int64_t i = 0; i < n; ++i) aptr[agg[i] + 1]++;
-    for (int64_t a = 0; a < na; ++a) aptr[a + 1] += aptr[a];
-    std::vector<int32_t> anodes(n);
-    {
-      std::vector<int64_t> pos(aptr.begin(), aptr.end() - 1);
-      for (int64_t i = 0; i < n; ++i) anodes[pos[agg[i]]++] = (int32_t)i;
-    }
-    if (lev == 0) {
-      // rigid-body rotations about ONE point (the node centroid) for the whole
-      // mesh: each aggregate's R then holds its coordinates of the same global
-      // modes, which the next level's tentative prolongator needs (rotations
-      // about a per-aggregate point span the same local space, but their R
-      // would describe a different, non-rigid field in every aggregate)
-      double cx = 0, cy = 0, cz = 0;
-      for (int64_t i = 0; i < nn; ++i) {
-        cx += c->X[i];
-        cy += c->Y[i];
-        cz += c->Z[i];
-      }
-      cx /= (double)nn;
-      cy /= (double)nn;
-      cz /= (double)nn;
-      for (int64_t i = 0; i < nn; ++i) {
-        const double x = c->X[i] - cx, y = c->Y[i] - cy, z = c->Z[i] - cz;
-        double* b = &Bcur[i * 18];
-        // rotations about x, y, z: columns 3, 4, 5 (rows: ux, uy, uz)
-        b[0 * 6 + 3] = 0;
-        b[1 * 6 + 3] = -z;
-        b[2 * 6 + 3] = y;
-        b[0 * 6 + 4] = z;
-        b[1 * 6 + 4] = 0;
-        b[2 * 6 + 4] = -x;
-        b[0 * 6 + 5] = -y;
-        b[1 * 6 + 5] = x;
-        b[2 * 6 + 5] = 0;
-        for (int d = 0; d < 3; ++d)
-          if ((c->fixmask[i] >> d) & 1u)
-            for (int q = 0; q < 6; ++q) b[d * 6 + q] = 0;
-      }
-    }
-    std::vector<double> T(n * bs * 6, 0.0), Bc(na * 36, 0.0);
-    tentative(aptr, anodes, bs, Bcur, T, Bc);
-
-    // P pattern (row i: the aggregates of its neighbours) and its lists: the
-    // operator blocks of row i per aggregate, in row order
-    std::vector<int32_t> p_ptr(n + 1, 0), p_col, p_row, pk;
-    std::vector<int64_t> pk_ptr(1, 0);
-    for (int64_t i = 0; i < n; ++i) {
-      trips.clear();
-      bool own = false;
-      for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-        trips.push_back({agg[cl[k]], k, 0});
-        own |= agg[cl[k]] == agg[i];
-      }
-      // every row gets a block in its own aggregate (a node without a diagonal
-      // block in the pattern -- an isolated, fixed node -- through an empty entry)
-      if (!own) trips.push_back({agg[i], -1, 0});
-      const size_t before = p_col.size();
-      group_row(trips, p_col, pk_ptr, pk, nullptr);
-      p_row.insert(p_row.end(), p_col.size() - before, (int32_t)i);
-      p_ptr[i + 1] = (int32_t)p_col.size();
-    }
-    const int64_t nnz_p = (int64_t)p_col.size();
-    // W = A P pattern (row i: the P columns of its neighbours) and its lists of
-    // (operator block, P block) pairs, in row and P order
-    std::vector<int32_t> w_ptr(n + 1, 0), w_col, wl_k, wl_p;
-    std::vector<int64_t> wl_ptr(1, 0);
-    for (int64_t i = 0; i < n; ++i) {
-      trips.clear();
-      for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
-        for (int32_t p = p_ptr[cl[k]]; p < p_ptr[cl[k] + 1]; ++p) trips.push_back({p_col[p], k, p});
-      group_row(trips, w_col, wl_ptr, wl_k, &wl_p);
-      w_ptr[i + 1] = (int32_t)w_col.size();
-    }
-    const int64_t nnz_w = (int64_t)w_col.size();
-    // P^T: per aggregate its P blocks, in row order
-    std::vector<int32_t> rt_ptr(na + 1, 0), rt(nnz_p), rt_row(nnz_p), rt_pos(nnz_p);
-    for (int64_t p = 0; p < nnz_p; ++p) rt_ptr[p_col[p] + 1]++;
-    for (int64_t a = 0; a < na; ++a) rt_ptr[a + 1] += rt_ptr[a];
-    {
-      std::vector<int32_t> pos(rt_ptr.begin(), rt_ptr.end() - 1);
-      for (int64_t p = 0; p < nnz_p; ++p) {
-        const int32_t e = pos[p_col[p]]++;
-        rt[e] = (int32_t)p;
-        rt_row[e] = p_row[p];
-        rt_pos[p] = e;
-      }
-    }
-    // next level: pattern of P^T W and its lists of (P block, W block) pairs
+  for (;;) {
+    host::SymbolicLevel S;
+    std::string err;
+    if (!host::symbolic_level(rp, cl, n, bs, Bcur, kMaxCoarseNodes, kSell6MinRows, kSell6Window,
+                              sell6_rows_per_wave(), S, &err))
+      return set_err(c, ONSAS_ERR_UNSUPPORTED, err);
+    const int64_t na = S.na;
     AmgLevel Lc;
     Lc.bs = 6;
     Lc.n = na;
-    Lc.row_ptr.assign(na + 1, 0);
-    std::vector<int32_t> gl_p, gl_w, diag(na, -1);
-    std::vector<int64_t> gl_ptr(1, 0);
-    for (int64_t I = 0; I < na; ++I) {
-      trips.clear();
-      bool has_diag = false;
-      for (int32_t e = rt_ptr[I]; e < rt_ptr[I + 1]; ++e) {
-        const int32_t p = rt[e], i = p_row[p];
-        for (int32_t w = w_ptr[i]; w < w_ptr[i + 1]; ++w) {
-          trips.push_back({w_col[w], p, w});
-          has_diag |= w_col[w] == I;
-        }
-      }
-      if (!has_diag) trips.push_back({(int32_t)I, -1, -1});  // zero block: unit diagonal later
-      const size_t before = Lc.col.size();
-      group_row(trips, Lc.col, gl_ptr, gl_p, &gl_w);
-      for (size_t k = before; k < Lc.col.size(); ++k)
-        if (Lc.col[k] == I) diag[I] = (int32_t)k;
-      Lc.row_ptr[I + 1] = (int32_t)Lc.col.size();
-      if (diag[I] < 0)
-        return set_err(c, ONSAS_ERR_UNSUPPORTED, "multigrid: coarse node without a diagonal block");
-    }
+    Lc.row_ptr = std::move(S.row_ptr);
+    Lc.col = std::move(S.col);
     Lc.nnzb = (int64_t)Lc.col.size();
-    // sliced-ELL layout of a large coarse operator (not the coarsest level):
-    // rows ordered by decreasing length within windows, rpw per slice
-    std::vector<int32_t> s6_ptr(1, 0), s6_col, s6_src, s6_row;
-    if (na > kMaxCoarseNodes && na >= kSell6MinRows) {
-      std::vector<int32_t> order(na);
-      for (int64_t I = 0; I < na; ++I) order[I] = (int32_t)I;
-      auto len = [&](int32_t I) { return Lc.row_ptr[I + 1] - Lc.row_ptr[I]; };
-      for (int64_t w0 = 0; w0 < na; w0 += kSell6Window)
-        std::stable_sort(order.begin() + w0, order.begin() + std::min<int64_t>(w0 + kSell6Window, na),
-                         [&](int32_t x, int32_t y) { return len(x) > len(y); });
-      const int rpw = sell6_rows_per_wave();
-      const int lpr = 64 / rpw;
-      Lc.rpw6 = rpw;
+    if (S.rpw > 0) {
+      Lc.rpw6 = S.rpw;
       Lc.ur6 = sell6_rounds_in_flight(f64);
-      for (int64_t r0 = 0; r0 < na; r0 += rpw) {
-        int32_t width = 0;
-        for (int r = 0; r < rpw; ++r) {
-          const int32_t I = r0 + r < na ? order[r0 + r] : -1;
-          s6_row.push_back(I);
-          if (I >= 0) width = std::max(width, (len(I) + lpr - 1) / lpr);
-        }
-        for (int32_t m = 0; m < width; ++m)
-          for (int l = 0; l < 64; ++l) {
-            const int32_t I = s6_row[r0 + (l & (rpw - 1))];
-            const int32_t slot = lpr * m + l / rpw;
-            const bool ok = I >= 0 && slot < len(I);
-            const int32_t k = ok ? Lc.row_ptr[I] + slot : -1;
-            s6_src.push_back(k);
-            s6_col.push_back(ok ? Lc.col[k] : 0);
-          }
-        s6_ptr.push_back(s6_ptr.back() + width);
-      }
-      Lc.nslices6 = (int64_t)s6_ptr.size() - 1;
-      Lc.nrounds6 = s6_ptr.back();
+      Lc.nslices6 = (int64_t)S.s6_ptr.size() - 1;
+      Lc.nrounds6 = S.s6_ptr.back();
     }
 
     // device arrays of the transfer (finer level) and of the coarse level
     AmgLevel& F = A.lev.back();
-    F.nnz_p = nnz_p;
-    F.nnz_w = nnz_w;
-    F.restrict_wide = nnz_p > (int64_t)kRestrictWide * na;
-    ONSAS_TRY(upload(c, &F.d_agg, agg));
-    ONSAS_TRY(f64 ? upload_kt<double>(c, &F.d_T, T) : upload_kt<float>(c, &F.d_T, T));
-    ONSAS_TRY(upload(c, &F.d_p_ptr, p_ptr));
-    ONSAS_TRY(upload(c, &F.d_p_col, p_col));
-    ONSAS_TRY(upload(c, &F.d_p_row, p_row));
-    ONSAS_TRY(upload(c, &F.d_pk_ptr, pk_ptr));
-    ONSAS_TRY(upload(c, &F.d_pk, pk));
-    ONSAS_TRY(upload(c, &F.d_rt_ptr, rt_ptr));
-    ONSAS_TRY(upload(c, &F.d_rt_row, rt_row));
-    ONSAS_TRY(upload(c, &F.d_rt_pos, rt_pos));
-    ONSAS_TRY(upload(c, &F.d_wl_ptr, wl_ptr));
-    ONSAS_TRY(upload(c, &F.d_wl_k, wl_k));
-    ONSAS_TRY(upload(c, &F.d_wl_p, wl_p));
-    ONSAS_TRY(upload(c, &F.d_gl_ptr, gl_ptr));
-    ONSAS_TRY(upload(c, &F.d_gl_p, gl_p));
-    ONSAS_TRY(upload(c, &F.d_gl_w, gl_w));
+    F.nnz_p = S.nnz_p;
+    F.nnz_w = S.nnz_w;
+    F.restrict_wide = S.nnz_p > (int64_t)kRestrictWide * na;
+    ONSAS_TRY(upload(c, &F.d_agg, S.agg));
+    ONSAS_TRY(f64 ? upload_kt<double>(c, &F.d_T, S.T) : upload_kt<float>(c, &F.d_T, S.T));
+    ONSAS_TRY(upload(c, &F.d_p_ptr, S.p_ptr));
+    ONSAS_TRY(upload(c, &F.d_p_col, S.p_col));
+    ONSAS_TRY(upload(c, &F.d_p_row, S.p_row));
+    ONSAS_TRY(upload(c, &F.d_pk_ptr, S.pk_ptr));
+    ONSAS_TRY(upload(c, &F.d_pk, S.pk));
+    ONSAS_TRY(upload(c, &F.d_rt_ptr, S.rt_ptr));
+    ONSAS_TRY(upload(c, &F.d_rt_row, S.rt_row));
+    ONSAS_TRY(upload(c, &F.d_rt_pos, S.rt_pos));
+    ONSAS_TRY(upload(c, &F.d_wl_ptr, S.wl_ptr));
+    ONSAS_TRY(upload(c, &F.d_wl_k, S.wl_k));
+    ONSAS_TRY(upload(c, &F.d_wl_p, S.wl_p));
+    ONSAS_TRY(upload(c, &F.d_gl_ptr, S.gl_ptr));
+    ONSAS_TRY(upload(c, &F.d_gl_p, S.gl_p));
+    ONSAS_TRY(upload(c, &F.d_gl_w, S.gl_w));
     const size_t pbytes = f64 ? sizeof(double) : sizeof(_Float16);  // P's value type
-    ONSAS_TRY(alloc(c, &F.d_P, pbytes * bs * 6 * (size_t)nnz_p));
-    ONSAS_TRY(alloc(c, &F.d_PT, pbytes * bs * 6 * (size_t)nnz_p));
-    ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)nnz_w));
+    ONSAS_TRY(alloc(c, &F.d_P, pbytes * bs * 6 * (size_t)S.nnz_p));
+    ONSAS_TRY(alloc(c, &F.d_PT, pbytes * bs * 6 * (size_t)S.nnz_p));
+    ONSAS_TRY(alloc(c, &F.d_W, c->kt * bs * 6 * (size_t)S.nnz_w));
     ONSAS_TRY(upload(c, &Lc.d_row_ptr, Lc.row_ptr));
     ONSAS_TRY(upload(c, &Lc.d_col, Lc.col));
-    ONSAS_TRY(upload(c, &Lc.d_diag, diag));
+    ONSAS_TRY(upload(c, &Lc.d_diag, S.diag));
     ONSAS_TRY(alloc(c, &Lc.d_A, c->kt * 36 * (size_t)Lc.nnzb));
     if (Lc.nslices6 > 0) {
-      ONSAS_TRY(upload(c, &Lc.d_s6_ptr, s6_ptr));
-      ONSAS_TRY(upload(c, &Lc.d_s6_col, s6_col));
-      ONSAS_TRY(upload(c, &Lc.d_s6_src, s6_src));
-      ONSAS_TRY(upload(c, &Lc.d_s6_row, s6_row));
+      ONSAS_TRY(upload(c, &Lc.d_s6_ptr, S.s6_ptr));
+      ONSAS_TRY(upload(c, &Lc.d_s6_col, S.s6_col));
+      ONSAS_TRY(upload(c, &Lc.d_s6_src, S.s6_src));
+      ONSAS_TRY(upload(c, &Lc.d_s6_row, S.s6_row));
       ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(double) : sizeof(uint16_t)) * 36 * 64 *
                                         (size_t)Lc.nrounds6));
     }
@@ -1362,7 +1167,7 @@ int amg_build(Ctx* c) {
     cl = A.lev.back().col;
     n = na;
     bs = 6;
-    Bcur.swap(Bc);
+    Bcur.swap(S.Bc);
   }
   for (void** v : {&A.lev[0].d_x, &A.lev[0].d_r, &A.lev[0].d_d0, &A.lev[0].d_d1, &A.lev[0].d_ev})
     ONSAS_TRY(alloc(c, v, c->kt * 3 * (size_t)nn));

@@ -8,6 +8,7 @@
 #include <numeric>
 
 #include "engine.hpp"
+#include "host_setup.hpp"
 
 namespace onsas {
 
@@ -89,49 +90,8 @@ int color_group(Ctx* c, ElemGroup& g) {
   return ONSAS_OK;
 }
 
-// Element order of the gather assembly.  Lane l of a gather wave handles row
-// r0 + l; for its q-th contribution to be next to lane l+1's in the state
-// planes, elements that are translates of each other must be adjacent.  Key:
-// the element's shape signature (node ids relative to its first node) and then
-// its first node: on a structured mesh every translate class is one run in
-// node order.  When the signatures do not repeat (unstructured meshes) the key
-// falls back to the smallest node id, which keeps the node-numbering locality.
 std::vector<int32_t> gather_order(const ElemGroup& g) {
-  const int nn_e = g.nnode;
-  const int64_t ne = g.ne;
-  std::vector<int32_t> order(ne);
-  std::iota(order.begin(), order.end(), 0);
-  auto sig_less = [&](int32_t x, int32_t y) {
-    const int32_t* cx = &g.conn[(size_t)x * nn_e];
-    const int32_t* cy = &g.conn[(size_t)y * nn_e];
-    for (int a = 1; a < nn_e; ++a) {
-      const int64_t dx = (int64_t)cx[a] - cx[0], dy = (int64_t)cy[a] - cy[0];
-      if (dx != dy) return dx < dy;
-    }
-    if (cx[0] != cy[0]) return cx[0] < cy[0];
-    return x < y;
-  };
-  std::sort(order.begin(), order.end(), sig_less);
-  int64_t classes = 1;
-  for (int64_t k = 1; k < ne; ++k) {
-    const int32_t* cx = &g.conn[(size_t)order[k - 1] * nn_e];
-    const int32_t* cy = &g.conn[(size_t)order[k] * nn_e];
-    for (int a = 1; a < nn_e; ++a)
-      if (cx[a] - cx[0] != cy[a] - cy[0]) {
-        ++classes;
-        break;
-      }
-  }
-  if (classes > 512) {
-    std::vector<int32_t> mn(ne);
-    for (int64_t e = 0; e < ne; ++e) {
-      const int32_t* ce = &g.conn[(size_t)e * nn_e];
-      mn[e] = *std::min_element(ce, ce + nn_e);
-    }
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return mn[x] < mn[y]; });
-  }
-  return order;
+  return host::gather_order({g.conn.data(), g.nnode, g.ne});
 }
 
 // Contribution lists of the gather assembly.  Block (r, k) of the tangent
@@ -257,51 +217,24 @@ int finalize_model(Ctx* c) {
   }
 
   // block pattern: every (node_a, node_b) pair of every element
-  std::vector<int64_t> cnt(nn + 1, 0);
-  for (auto& g : c->groups)
-    for (int64_t e = 0; e < g.ne; ++e)
-      for (int a = 0; a < g.nnode; ++a) cnt[g.conn[e * g.nnode + a] + 1] += g.nnode;
-  for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int32_t> cand(cnt[nn]);
   {
-    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-    for (auto& g : c->groups)
-      for (int64_t e = 0; e < g.ne; ++e)
-        for (int a = 0; a < g.nnode; ++a) {
-          int32_t r = g.conn[e * g.nnode + a];
-          for (int b = 0; b < g.nnode; ++b) cand[pos[r]++] = g.conn[e * g.nnode + b];
-        }
-  }
-  c->row_ptr.assign(nn + 1, 0);
-  for (int64_t i = 0; i < nn; ++i) {
-    auto b = cand.begin() + cnt[i], e = cand.begin() + cnt[i + 1];
-    std::sort(b, e);
-    auto u = std::unique(b, e);
-    c->row_ptr[i + 1] = c->row_ptr[i] + (int32_t)(u - b);
+    std::vector<host::ConnView> views;
+    for (const auto& g : c->groups) views.push_back({g.conn.data(), g.nnode, g.ne});
+    host::Pattern P = host::block_pattern(nn, views);
+    c->row_ptr = std::move(P.row_ptr);
+    c->col = std::move(P.col);
+    c->diag = std::move(P.diag);
   }
   c->nnzb = c->row_ptr[nn];
   if (c->nnzb >= (int64_t(1) << 31) / 9)
     return set_err(c, ONSAS_ERR_UNSUPPORTED, "tangent too large for 32-bit slot indices");
-  c->col.resize(c->nnzb);
-  c->diag.assign(nn, -1);
-  for (int64_t i = 0; i < nn; ++i) {
-    std::copy(cand.begin() + cnt[i], cand.begin() + cnt[i] + (c->row_ptr[i + 1] - c->row_ptr[i]),
-              c->col.begin() + c->row_ptr[i]);
-  }
-  for (int64_t i = 0; i < nn; ++i) {
-    auto b = c->col.begin() + c->row_ptr[i], e = c->col.begin() + c->row_ptr[i + 1];
-    auto it = std::lower_bound(b, e, (int32_t)i);
-    if (it != e && *it == i) c->diag[i] = (int32_t)(it - c->col.begin());
-  }
   // nodes that no element touches get no diagonal block: they must be fixed
   for (int64_t i = 0; i < nn; ++i)
     if (c->diag[i] < 0 && c->fixmask[i] != 7)
       return set_err(c, ONSAS_ERR_ARG,
                      "node " + std::to_string(i) + " belongs to no element and is not fixed");
-  auto slot_of = [&](int32_t r, int32_t cc) -> int32_t {
-    auto b = c->col.begin() + c->row_ptr[r], e = c->col.begin() + c->row_ptr[r + 1];
-    return (int32_t)(std::lower_bound(b, e, cc) - c->col.begin());
-  };
+  host::Pattern pat{c->row_ptr, c->col, {}};
+  auto slot_of = [&](int32_t r, int32_t cc) { return host::slot_of(pat, r, cc); };
 
   std::vector<std::vector<int32_t>> slots_h(c->groups.size());
 

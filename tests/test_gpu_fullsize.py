@@ -7,10 +7,11 @@ that do not need the oracle to run at that size:
     ||r|| / ||f_ext|| <= 1e-8, and the support reactions balance the load
     (the internal forces of any displacement field sum to zero, so the pinned
     nodes' f_int is minus the applied load at equilibrium).
-  * thick cylinder 24 x 192 x 96 (2,654,208 tets), LinearStaticAnalysis in
-    mixed mode: radial displacement of the inner and outer walls vs the
-    plane-strain Lame solution (examples/cylinder_internal_pressure) within
-    the discretisation error, and the solution is exactly linear in the load.
+  * thick cylinder 24 x 192 x 96 (2,654,208 tets), LinearStaticAnalysis:
+    mixed vs all-FP64 displacements agree to 1e-6 of max |U| at both steps;
+    radial displacement of the inner and outer walls vs the plane-strain Lame
+    solution (examples/cylinder_internal_pressure) within the discretisation
+    error, and the solution is exactly linear in the load.
   * identification cube 48^3 (663,552 per-element shear moduli): the adjoint
     gradient vs a central finite difference of the misfit along a random
     direction over all parameters, through warm-started Newton solves.
@@ -74,12 +75,23 @@ def test_truss_lattice_41_mixed_vs_fp64(pc):
 def test_cylinder_full_size_lame(pc):
     print(f"[gpu] cylinder 24x192x96 full size, {pc}", flush=True)
     s, sm = workloads.thick_cylinder()
-    t0 = time.perf_counter()
-    sol = solve(LinearStaticAnalysis(s, NSTEPS=2), None,
-                MI355XBackend(precision="mixed", maxiter=100000, store_stresses=False,
-                              preconditioner=pc))
-    print(f"  solve {time.perf_counter() - t0:.2f} s, inner {sum(d['lin_iters'] for d in sol.newton_log)}",
-          flush=True)
+    sols = {}
+    for prec in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        sols[prec] = solve(LinearStaticAnalysis(s, NSTEPS=2), None,
+                           MI355XBackend(precision=prec, maxiter=100000, store_stresses=False,
+                                         preconditioner=pc))
+        print(f"  {prec} solve {time.perf_counter() - t0:.2f} s, inner "
+              f"{sum(d['lin_iters'] for d in sols[prec].newton_log)}", flush=True)
+        sols[prec].analysis.state.close()
+    # mixed (FP32 solve refined against the FP64 residual) vs all-FP64, per step:
+    # a check that sees solve errors far below Lame's discretisation error
+    for k in range(2):
+        scale = np.abs(sols["fp64"].U[k]).max()
+        d = np.abs(sols["mixed"].U[k] - sols["fp64"].U[k]).max() / scale
+        print(f"  step {k + 1}: mixed vs fp64 max rel difference {d:.2e}", flush=True)
+        assert d <= 1e-6, (k, d)
+    sol = sols["mixed"]
     U = sol.U[-1].reshape(-1, 3)
     r = np.linalg.norm(sm.coords[:, :2], axis=1)
     ur = (U[:, 0] * sm.coords[:, 0] + U[:, 1] * sm.coords[:, 1]) / r
@@ -125,3 +137,60 @@ def test_identification_cube_48_gradient():
     print(f"  directional: fd {fd:.8e} adjoint {ad:.8e} rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
     assert abs(fd - ad) <= 1e-5 * abs(ad)
     sol.analysis.state.close()
+
+
+def test_identification_cube_48_single_parameter_fd():
+    """The identification cube's 32 seeded single-parameter checks (h = 1e-4
+    mu_e, central differences): the mixed-mode adjoint gradient vs finite
+    differences of J through FP64 Newton re-solves from the converged state,
+    each driven to the FP64 floor of the residual.  A single element moves J by
+    about 1e-9 of its value, so the differences need the re-solves' U
+    difference to ~1e-14 of U; the unperturbed re-solve measures that floor."""
+    print("[gpu] identification cube 48^3: 32 single-parameter FDs", flush=True)
+    alg = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-16, rel_res_force_tol=1e-12, max_iter=40))
+    floor = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-17, rel_res_force_tol=1e-16,
+                                              max_iter=6))
+    be = MI355XBackend(precision="mixed", maxiter=50000, store_stresses=False,
+                       preconditioner="amg", amg_refresh="once")
+    be64 = MI355XBackend(precision="fp64", maxiter=50000, store_stresses=False,
+                         preconditioner="amg", amg_refresh="once", rtol=1e-14)
+    s_true, sm, p, G_true, obs = workloads.identification_cube(n=48)
+    sol_true = solve(NonLinearStaticAnalysis(s_true, NSTEPS=10), alg, be)
+    rng = np.random.default_rng(5)
+    target = sol_true.U[-1].reshape(-1, 3)[obs] + rng.normal(scale=1e-4, size=(len(obs), 3))
+    sol_true.analysis.state.close()
+    s, sm, p, _, _ = workloads.identification_cube(n=48)
+    G0 = np.full(len(sm.tets), 3.8e5)
+    next(iter(s.materials))[0].G = G0
+    sol, pullback = rrule(solve, NonLinearStaticAnalysis(s, NSTEPS=10), alg, be)
+    _, dJdU = misfit(sol.U[-1], obs, target)
+    g = pullback(dJdU)["tets"]
+    sol.analysis.state.close()
+    s64, _, _, _, _ = workloads.identification_cube(n=48)
+    next(iter(s64.materials))[0].G = G0
+    sol64 = solve(NonLinearStaticAnalysis(s64, NSTEPS=10), alg, be64)
+
+    def J(G):
+        sp = resolve(sol64, G, floor, be64)
+        return misfit(sp.U[-1], obs, target)[0], sp.records[-1]["dr_rel"]
+
+    J0, r0 = J(G0)
+    J0b, _ = J(G0)
+    print(f"  J {J0:.12e}, unperturbed re-solves differ by {abs(J0b - J0) / J0:.1e} of J, "
+          f"final residual {r0:.1e}", flush=True)
+    idx = np.random.default_rng(6).choice(len(G0), size=32, replace=False)
+    gmax = np.abs(g).max()
+    errs = []
+    for e in idx:
+        h = 1e-4 * G0[e]
+        Gp, Gm = G0.copy(), G0.copy()
+        Gp[e] += h
+        Gm[e] -= h
+        fd = (J(Gp)[0] - J(Gm)[0]) / (2 * h)
+        errs.append((abs(fd - g[e]) / abs(g[e]), abs(fd - g[e]) / gmax))
+        print(f"  element {e}: adjoint {g[e]:.8e} fd {fd:.8e} rel {errs[-1][0]:.2e} "
+              f"(of max |g| {errs[-1][1]:.2e})", flush=True)
+    rel = np.array([x[0] for x in errs])
+    print(f"  median rel {np.median(rel):.2e}, max rel {rel.max():.2e}", flush=True)
+    assert max(x[1] for x in errs) <= 1e-5
+    sol64.analysis.state.close()

@@ -94,14 +94,17 @@ def test_tet_element_outputs(oracle, prec, mat):
     c.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("model", [0, 1])
-def test_truss_element_outputs(oracle, model):
-    print(f"[gpu] truss element outputs model={model}", flush=True)
+def test_truss_element_outputs(oracle, model, prec):
+    """Batch internal_forces of trusses vs the oracle; the element outputs are
+    evaluated in FP64 in both precision modes."""
+    print(f"[gpu] truss element outputs model={model} {prec}", flush=True)
     tl = meshes.truss_lattice(2, h=0.5, jitter=0.05)
     rng = np.random.default_rng(3)
     U = rng.normal(scale=0.01, size=3 * tl.nn)
     A = rng.uniform(0.5, 1.5, size=len(tl.trusses))
-    c = _ctx("fp64", tl.coords, trusses=tl.trusses, strain=model, E=2.0, A=A)
+    c = _ctx(prec, tl.coords, trusses=tl.trusses, strain=model, E=2.0, A=A)
     c.set_U(U)
     f, K, s, e = c.element_forces(0)
     for k, (a, b) in enumerate(tl.trusses):

@@ -190,3 +190,63 @@ def test_point_location_matches_brute_force():
     np.testing.assert_allclose(interpolate(h, f), h.in_mesh_points @ np.array([1.0, -2.0, 0.5]) + 3.0,
                                rtol=0, atol=1e-12)
     st.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cylinder_mid_size_vs_oracle(oracle, prec):
+    """A mid-size cylinder (6 x 48 x 12 cells, 20,736 tets, 13,104 dofs): the
+    GPU LinearStaticAnalysis vs the oracle's FP64 solve at 1e-8 of max |U|."""
+    print(f"[gpu] cylinder 6x48x12 linear analysis {prec}", flush=True)
+    s, sm = workloads.thick_cylinder(nr=6, ntheta=48, nz=12, Lz=0.1)
+    sol = solve(LinearStaticAnalysis(s, NSTEPS=1), None,
+                MI355XBackend(precision=prec, maxiter=50000, preconditioner="amg"))
+    m = oracle_model(oracle, s)
+    _, Us, _ = m.newton(fext_steps(oracle, s, [1.0]), rel_U_tol=1e-16, rel_res_tol=1e-13,
+                        max_iter=30, cg_reltol=1e-14)
+    err = np.abs(sol.U[0] - Us[0]).max() / np.abs(Us[0]).max()
+    print(f"  max rel difference {err:.2e}", flush=True)
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_stored_stress_is_at_the_last_assembly(oracle, monkeypatch, prec):
+    """store! keeps the sigma/eps of the step's last assemble!, which precedes the
+    final U += dU (NonLinearStaticAnalyses.jl:90-98, StaticAnalyses.jl:157-174).
+    With a loose Newton tolerance the last update is visible: the stored P and C
+    must equal the oracle's element outputs at the U of that last assembly
+    (U_k), and differ from those at the stored U (U_k + dU)."""
+    from onsas_amd import analysis
+    print(f"[gpu] stored stress at the last assembly {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=4)
+    u_before = []
+    orig = analysis.step_
+
+    def recording_step(sa, alg, linear_solver=None, sol=None):
+        u_before.append((sa.current_step, sa.state.ctx.get_U()))
+        return orig(sa, alg, linear_solver, sol)
+
+    monkeypatch.setattr(analysis, "step_", recording_step)
+    alg = NewtonRaphson(ConvergenceSettings(rel_U_tol=1e-16, rel_res_force_tol=1e-3, max_iter=30))
+    sol = solve(NonLinearStaticAnalysis(s, NSTEPS=3), alg,
+                MI355XBackend(precision=prec, store_stresses=True))
+    (mat, es), = list(s.materials)
+    K, G = mat.params()
+    seen_gap = 0.0
+    for k in range(3):
+        Uk = [u for step, u in u_before if step == k + 1][-1]  # U at the step's last assembly
+        Uk1 = sol.U[k]
+        P, Cm = sol.stress[k][0], sol.strain[k][0]
+        for e in range(0, len(sm.tets), 37):
+            tet = sm.tets[e]
+            X = sm.coords[tet].reshape(-1)
+            dofs = (3 * tet[:, None] + np.arange(3)).reshape(-1)
+            _, _, Po, Co = oracle.tet_hyperelastic(oracle.MAT_NEOHOOKEAN, K, G, X, Uk[dofs])
+            _, _, P1, _ = oracle.tet_hyperelastic(oracle.MAT_NEOHOOKEAN, K, G, X, Uk1[dofs])
+            sc = np.abs(Po).max()
+            np.testing.assert_allclose(P[e], Po, rtol=0, atol=1e-10 * sc)
+            np.testing.assert_allclose(Cm[e], Co, rtol=0, atol=1e-12)
+            seen_gap = max(seen_gap, np.abs(P1 - Po).max() / sc)
+    print(f"  largest |P(U_k + dU) - P(U_k)| / |P| over the checked elements {seen_gap:.2e}",
+          flush=True)
+    assert seen_gap > 1e-7  # the test can tell U_k from U_k + dU
+    sol.analysis.state.close()

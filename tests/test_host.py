@@ -201,3 +201,21 @@ def test_integration_c_example_compiles(tmp_path):
                         "-I", os.path.join(root, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_host_setup_under_sanitizers(tmp_path):
+    """The host algorithms of the setup (block pattern, slot maps, gather order,
+    the multigrid's aggregation, tentative prolongator, sum lists and
+    sliced-ELL layout: onsas.jl_amd/csrc/host_setup.hpp) built with g++ under
+    AddressSanitizer and UndefinedBehaviorSanitizer and run on structured and
+    scrambled meshes with invariant checks (tests/host/host_setup_check.cpp)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "host_setup_check")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+                        "-fno-sanitize-recover=all", "-I", os.path.join(root, "onsas.jl_amd", "csrc"),
+                        os.path.join(root, "tests", "host", "host_setup_check.cpp"), "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok ") == 4, r.stdout
