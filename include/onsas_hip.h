@@ -267,6 +267,15 @@ int onsas_time_level_products(onsas_ctx* ctx, int32_t level, int32_t launches,
                               double* ms_per_launch);
 /* y = K x with the assembled tangent (n values each). */
 int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y);
+/* The same product without allocating anything.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  x and
+ * y pass through a fixed 32 MiB device array that each GPU the library runs on
+ * holds until the process ends, so n is limited to 2,097,152 dofs (699,050
+ * nodes; larger models get ONSAS_ERR_ARG).  Calls that overlap on one GPU
+ * share that array and race; the result is otherwise that of
+ * onsas_tangent_matvec. */
+int onsas_tangent_matvec_noalloc(onsas_ctx* ctx, const double* x, double* y);
 
 /* Batch internal_forces(mat, e, u_e) over one element group at the current U
  * (src/Entities/Entities.jl:173-218; Tetrahedrons.jl:186-266;

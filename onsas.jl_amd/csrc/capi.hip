@@ -5,10 +5,21 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "engine.hpp"
+
+// ---- device memory of onsas_tangent_matvec_noalloc -----------------------------
+// x and y of that entry point, kNoAllocMaxDofs doubles each, in one file-scope
+// array (32 MiB) that every GPU the library runs on holds until the process
+// ends.  Host code never passes `scratch` itself to a kernel or a copy: the
+// array's name is a host-side stand-in, not a device address.  onsas_create
+// looks the address up once per device (hipGetSymbolAddress, with that device
+// current) and every later use goes through that pointer.
+static constexpr int64_t kNoAllocMaxDofs = int64_t(1) << 21;  // 2,097,152 dofs
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxDofs];
 
 namespace onsas {
 
@@ -27,6 +38,23 @@ namespace {
 
 std::string g_create_err;
 
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device id: the device address of `scratch`
+
+// The device address of `scratch` on device `dev`, which must be current:
+// looked up on the first call for that device, then reused.
+hipError_t scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 Ctx* C(onsas_ctx* ctx) { return ctx ? &ctx->impl : nullptr; }
 
@@ -235,6 +263,12 @@ int onsas_create(int32_t precision, onsas_ctx** out) {
   Ctx* c = &ctx->impl;
   c->precision = precision;
   (void)hipGetDevice(&c->device);
+  e = scratch_address(c->device, &c->d_noalloc);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    delete ctx;
+    return ONSAS_ERR_HIP;
+  }
   e = hipStreamCreate(&c->stream);  // default priority
   if (e != hipSuccess) {
     g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -661,6 +695,27 @@ int onsas_tangent_matvec(onsas_ctx* ctx, const double* x, double* y) {
   ONSAS_HIP_TRY(c, hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return ONSAS_OK;
+}
+
+// The same product with no allocation: x and y go through the device's
+// `scratch` array (via the pointer onsas_create looked up) and the context's
+// pinned stage, both allocated before this call.  Calls that overlap on one GPU
+// share that array and race.
+int onsas_tangent_matvec_noalloc(onsas_ctx* ctx, const double* x, double* y) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!x || !y) return set_err(c, ONSAS_ERR_ARG, "NULL vector");
+  const int64_t n = 3 * c->nn;
+  if (n > kNoAllocMaxDofs)
+    return set_err(c, ONSAS_ERR_ARG,
+                   "onsas_tangent_matvec_noalloc: " + std::to_string(n) +
+                       " dofs exceed its limit of 2097152 (699050 nodes); use onsas_tangent_matvec");
+  if (!c->d_noalloc) return set_err(c, ONSAS_ERR_STATE, "no scratch address for this device");
+  double* dx = c->d_noalloc;
+  double* dy = c->d_noalloc + kNoAllocMaxDofs;
+  ONSAS_TRY(upload_state(c, dx, x));  // every x value is overwritten: nothing to clear
+  ONSAS_TRY(onsas::launch_matvec(c, dx, dy));  // writes every y value
+  return download_state(c, y, dy);
 }
 
 namespace {

@@ -259,6 +259,7 @@ struct Ctx {
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
   uint8_t* d_fixmask = nullptr;
   int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_diag = nullptr;
+  double* d_noalloc = nullptr;  // this device's `scratch` (capi.hip), not owned
   void* d_K = nullptr;     // nnzb*9 KT
   void* d_Dinv = nullptr;  // nn*9 KT
   double *d_U = nullptr, *d_Uasm = nullptr, *d_fint = nullptr, *d_fext = nullptr, *d_res = nullptr;

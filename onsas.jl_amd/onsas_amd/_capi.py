@@ -34,7 +34,7 @@ EXPORTED = (
     "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_snapshot_U",
     "onsas_read_snapshot", "onsas_set_U", "onsas_get_fext",
     "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
-    "onsas_tangent_matvec", "onsas_element_forces",
+    "onsas_tangent_matvec", "onsas_tangent_matvec_noalloc", "onsas_element_forces",
     "onsas_get_stress_strain",
     "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
     "onsas_get_timings", "onsas_reset_timings", "onsas_inject_error", "onsas_amg_levels",
@@ -121,6 +121,7 @@ def load():
         "onsas_get_pattern": ([_vp, _vp, _vp], C.c_int),
         "onsas_get_tangent": ([_vp, _vp], C.c_int),
         "onsas_tangent_matvec": ([_vp, _vp, _vp], C.c_int),
+        "onsas_tangent_matvec_noalloc": ([_vp, _vp, _vp], C.c_int),
         "onsas_element_forces": ([_vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
         "onsas_get_stress_strain": ([_vp, C.c_int32, C.c_int32, _vp, _vp], C.c_int),
         "onsas_linear_solve": ([_vp, _vp, _vp, _i32p, _dp], C.c_int),
@@ -376,6 +377,14 @@ class Context:
         xv = _f64(x, 3 * self.nn)
         y = np.empty_like(xv)
         self._check(self.L.onsas_tangent_matvec(self.h, xv.ctypes.data, y.ctypes.data))
+        return y
+
+    def matvec_noalloc(self, x):
+        """K x through onsas_tangent_matvec_noalloc (no device allocation; at
+        most 2,097,152 dofs; calls overlapping on one GPU race)."""
+        xv = _f64(x, 3 * self.nn)
+        y = np.empty_like(xv)
+        self._check(self.L.onsas_tangent_matvec_noalloc(self.h, xv.ctypes.data, y.ctypes.data))
         return y
 
     def element_forces(self, group, want_K=True):
