@@ -199,8 +199,11 @@ def kernel_rooflines(r, kt):
         vb = 2 if kt == 4 else 8
         b = int(e["nnzb"] * (36 * vb + 4) + 49.5 * kt * e["n"])
         ms = e["batch_ms_per_launch"]
+        rpw = os.environ.get("ONSAS_SELL6_RPW", "8")
+        ur = os.environ.get("ONSAS_SELL6_ROUNDS", "2" if kt == 4 else "1")
         out["k_sell6_cheb"] = dict(
-            kernel=f"k_sell6_cheb<{'float, unsigned short' if kt == 4 else 'double, double'}, 4>",
+            kernel=f"k_sell6_cheb<{'float, unsigned short' if kt == 4 else 'double, double'}, "
+                   f"{rpw}, {ur}>",
             what="level-1 sliced-ELL 6x6 operator product + Chebyshev epilogue",
             bytes_per_launch=b, ms_per_launch=ms, launches_per_pcg_iter=4,
             GBps=b / (ms * 1e-3) / 1e9)

@@ -50,6 +50,9 @@ constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
 constexpr int64_t kSell6MinRows = 2048;  // smaller levels: one wave per block row
 constexpr int kSell6Window = 256;     // rows sorted by length within windows of this size
 constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 4 waves restrict
+// LDS of a sliced-ELL coarse product's workgroup for the x values of its column
+// window (2,048 nodes in FP32, 1,024 in FP64)
+constexpr size_t kX6Lds = 48 * 1024;
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -610,90 +613,165 @@ struct Sell6Layout {
 };
 
 // Large levels >= 1 on their sliced-ELL copy: one wave per slice of RPW rows,
-// 64 / RPW lanes per row (lane l serves row l % RPW).  A lane issues the loads
-// of UR rounds at once (column indices, the 36 values of each round, then the
-// gathers of x), with the round index clamped to the slice's last round, before
-// any product; the products of rounds past the end are skipped (a wave-uniform
-// test), so every sum runs over the rounds and the 36 values in order as a
-// one-round loop would.  A slice of up to UR rounds thus costs one index ->
-// gather chain instead of one per round.  After the reduction over the lanes of
-// a row, lanes l / RPW < 6 finish that component of row s6_row[RPW s + l % RPW]
+// 64 / RPW lanes per row (lane l serves row l % RPW), kSell6Waves slices per
+// workgroup.  The workgroup first copies the x values of its column window
+// (host-computed: lo, span <= kX6Lds bytes) into LDS with coalesced 16-byte
+// loads, issued with the first pass's column indices, values and row ids, so a
+// lane's gathers of x come from LDS (a column outside the window, which only
+// an irregular level has, is read from global memory).  A lane issues the
+// loads of UR rounds at once, with the round index clamped to the slice's last
+// round; rounds past the end multiply by exact zeros, so every sum runs over
+// the rounds and the 36 values in order as a one-round loop would.  The row
+// operands of the epilogue (b, D^-1, v, x_base) are loaded before the barrier
+// and used after the products.  After the reduction over the lanes of a row,
+// lanes l / RPW < 6 finish that component of row s6_row[RPW s + l % RPW]
 // (rows sorted by length within windows; -1: none).
+// Waves per workgroup: 16 with the LDS window (FP32 x), 4 without it (FP64 x:
+// its 36 values a round need the registers that the window's copy would take).
+constexpr int kSell6Waves = 16;
+template <typename KT>
+constexpr bool sell6_window() { return sizeof(KT) == 4; }
+template <typename KT>
+constexpr int sell6_waves() { return sell6_window<KT>() ? kSell6Waves : 4; }
 template <typename KT, typename VT, int RPW, int UR>
-__global__ __launch_bounds__(256) void k_sell6_cheb(
+__device__ inline void sell6_load_pass(const int32_t* __restrict__ s6_col, const VT* __restrict__ A6,
+                                       int32_t qb, int32_t qlast, int l, int32_t (&cj)[UR],
+                                       VT (&av)[UR][36]) {
+  using LY = Sell6Layout<VT>;
+#pragma unroll
+  for (int k = 0; k < UR; ++k) cj[k] = s6_col[64 * (int64_t)min(qb + k, qlast) + l];
+#pragma unroll
+  for (int k = 0; k < UR; ++k) {  // unconditional loads (clamped rounds)
+    const VT* a = A6 + LY::kRound * (int64_t)min(qb + k, qlast);
+#pragma unroll
+    for (int ch = 0; ch < LY::kFull; ++ch) {
+      const VT* ap = (const VT*)__builtin_assume_aligned(a + LY::offset(ch * LY::kVPC, l), 16);
+#pragma unroll
+      for (int j = 0; j < LY::kVPC; ++j) av[k][ch * LY::kVPC + j] = ap[j];
+    }
+    if constexpr (LY::kRem > 0) {
+      const VT* ap = (const VT*)__builtin_assume_aligned(a + LY::offset(LY::kFull * LY::kVPC, l),
+                                                         LY::kRem * sizeof(VT));
+#pragma unroll
+      for (int j = 0; j < LY::kRem; ++j) av[k][LY::kFull * LY::kVPC + j] = ap[j];
+    }
+  }
+}
+
+// a clamped round past the slice's end multiplies by exact zeros: the sum is
+// unchanged, and no branch lets the loads sink below a test
+template <typename KT, typename VT, int UR>
+__device__ inline void sell6_accumulate(const VT (&av)[UR][36], const KT (&xv)[UR][6], int32_t qb,
+                                        int32_t q1, KT (&acc)[6]) {
+#pragma unroll
+  for (int k = 0; k < UR; ++k) {
+    const bool ok = qb + k < q1;
+#pragma unroll
+    for (int e = 0; e < 36; ++e) acc[e / 6] += ldv<KT>(av[k][e]) * (ok ? xv[k][e % 6] : KT(0));
+  }
+}
+
+template <typename KT, typename VT, int RPW, int UR>
+__global__ __launch_bounds__(64 * sell6_waves<KT>()) void k_sell6_cheb(
     const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
-    const int32_t* __restrict__ s6_row, const VT* __restrict__ A6, const KT* __restrict__ v,
-    const KT* b, KT* r_out,
+    const int32_t* __restrict__ s6_row, const int32_t* __restrict__ s6_win,
+    const VT* __restrict__ A6, const KT* __restrict__ v, const KT* b, KT* r_out,
     const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out, const KT* x_base,
     KT* x_out, int64_t nslices, int64_t n, const PcgState* st) {
   static_assert(RPW == 2 || RPW == 4 || RPW == 8, "6 components need >= 8 lanes per row");
   static_assert(UR >= 1 && UR <= 4, "1 to 4 rounds in flight (registers: 36 values each)");
-  using LY = Sell6Layout<VT>;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr bool kWin = sell6_window<KT>();
+  constexpr int kThreads = 64 * sell6_waves<KT>();
+  constexpr int kWinUnits = kWin ? (int)(kX6Lds / 16) : 1;  // 16-byte units of the window
+  constexpr int kPerThread = (kWinUnits + kThreads - 1) / kThreads;
+  __shared__ __align__(16) uint4 xs4[kWinUnits];
+  const KT* xs = (const KT*)xs4;
+  const int64_t s0 = (int64_t)blockIdx.x * sell6_waves<KT>() + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
-  if (s >= nslices) return;
-  // the epilogue's row operands first (see k_sell_cheb)
-  const int64_t I = s6_row[RPW * s + (l & (RPW - 1))];
-  const int u = l / RPW;
-  const int64_t Ic = I >= 0 ? I : 0;
-  const int uc = u < 6 ? u : 0;
-  KT bv[6] = {0, 0, 0, 0, 0, 0}, dm[6], vu = 0, xb = 0;
-  if (b)
-#pragma unroll
-    for (int m = 0; m < 6; ++m) bv[m] = b[6 * Ic + m];
-  if (d_out || x_out) {
-#pragma unroll
-    for (int m = 0; m < 6; ++m) dm[m] = Dinv[36 * Ic + 6 * uc + m];
-    vu = v[6 * Ic + uc];
-    if (x_base) xb = x_base[6 * Ic + uc];
-  }
+  // every wave of the workgroup reaches the barrier: a wave past the last slice
+  // reads that slice's header, runs its rounds as zeros and writes nothing
+  const bool live = s0 < nslices;
+  const int64_t s = live ? s0 : nslices - 1;
   const int32_t su = __builtin_amdgcn_readfirstlane((int32_t)s);  // wave-uniform: scalar loads
-  const int32_t q0 = s6_ptr[su], q1 = s6_ptr[su + 1];
-  if (pcg_stopped(st)) return;
+  const int32_t q0 = s6_ptr[su], q1 = live ? s6_ptr[su + 1] : q0;
+  const int32_t lo = kWin ? s6_win[2 * blockIdx.x] : 0, span = kWin ? s6_win[2 * blockIdx.x + 1] : 0;
+  if (pcg_stopped(st)) return;  // the same for every wave of the launch
+  // rounds are clamped to qlast (a slice has at least one round; an empty one
+  // reads a neighbouring round and adds exact zeros)
+  const int32_t qlast = max(max(q1, q0 + 1) - 1, 0);
+  const int u = l / RPW;
+  const int uc = u < 6 ? u : 0;
+  // optional operands as always-valid pointers (selected on the kernel
+  // arguments), so the first pass has no branch that would wait at a join
+  const KT* bp = b ? b : v;
+  const KT* xbp = x_base ? x_base : v;
   KT acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t qb = q0; qb < q1; qb += UR) {
-    int32_t cj[UR];
+  // first pass: row id, columns and values, then the window (all independent)
+  const int64_t I = s6_row[RPW * s + (l & (RPW - 1))];
+  int32_t cj[UR];
+  VT av[UR][36];
+  sell6_load_pass<KT, VT, RPW, UR>(s6_col, A6, q0, qlast, l, cj, av);
+  const int nunits = (int)(((int64_t)span * 6 * (int64_t)sizeof(KT)) / 16);
+  const uint4* wsrc = (const uint4*)(v + 6 * (int64_t)lo);  // 16-byte aligned: lo even
+  uint4 wv[kPerThread];
+  // the window's last 8 bytes when its length is not a multiple of 16 bytes
+  // (a span that ends at the last node)
+  const int tail8 = (int)((((int64_t)span * 6 * (int64_t)sizeof(KT)) % 16) / 8);
+  uint2 wt = make_uint2(0, 0);
+  if constexpr (kWin) {
 #pragma unroll
-    for (int k = 0; k < UR; ++k) cj[k] = s6_col[64 * (int64_t)min(qb + k, q1 - 1) + l];
-    VT av[UR][36];
-#pragma unroll
-    for (int k = 0; k < UR; ++k) {  // unconditional loads (clamped rounds)
-      const VT* a = A6 + LY::kRound * (int64_t)min(qb + k, q1 - 1);
-#pragma unroll
-      for (int ch = 0; ch < LY::kFull; ++ch) {
-        const VT* ap = (const VT*)__builtin_assume_aligned(a + LY::offset(ch * LY::kVPC, l), 16);
-#pragma unroll
-        for (int j = 0; j < LY::kVPC; ++j) av[k][ch * LY::kVPC + j] = ap[j];
-      }
-      if constexpr (LY::kRem > 0) {
-        const VT* ap = (const VT*)__builtin_assume_aligned(
-            a + LY::offset(LY::kFull * LY::kVPC, l), LY::kRem * sizeof(VT));
-#pragma unroll
-        for (int j = 0; j < LY::kRem; ++j) av[k][LY::kFull * LY::kVPC + j] = ap[j];
-      }
+    for (int j = 0; j < kPerThread; ++j) {
+      const int t = (int)threadIdx.x + kThreads * j;
+      wv[j] = wsrc[t < nunits ? t : 0];
     }
-    KT xv[UR][6];
+    if (tail8 && threadIdx.x == 0) wt = ((const uint2*)wsrc)[2 * nunits];
+  }
+  const int64_t Ic = I >= 0 ? I : 0;
+  KT bv[6], dm[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) bv[m] = bp[6 * Ic + m];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) dm[m] = Dinv[36 * Ic + 6 * uc + m];
+  const KT vu = v[6 * Ic + uc];
+  const KT xb0 = xbp[6 * Ic + uc];
+  if constexpr (kWin) {
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      const int t = (int)threadIdx.x + kThreads * j;
+      if (t < nunits) xs4[t] = wv[j];
+    }
+    if (tail8 && threadIdx.x == 0) ((uint2*)xs4)[2 * nunits] = wt;
+    __syncthreads();
+  }
+  KT xv[UR][6];
+#pragma unroll
+  for (int k = 0; k < UR; ++k) {
+    const uint32_t cw = (uint32_t)(cj[k] - lo);
+    const KT* xj = cw < (uint32_t)span ? xs + 6 * cw : v + 6 * (int64_t)cj[k];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) xv[k][c] = xj[c];
+  }
+  sell6_accumulate<KT, VT, UR>(av, xv, q0, q1, acc);
+  for (int32_t qb = q0 + UR; qb < q1; qb += UR) {
+    sell6_load_pass<KT, VT, RPW, UR>(s6_col, A6, qb, qlast, l, cj, av);
 #pragma unroll
     for (int k = 0; k < UR; ++k) {
-      const KT* xj = (const KT*)__builtin_assume_aligned(v + 6 * (int64_t)cj[k], 8);
+      const uint32_t cw = (uint32_t)(cj[k] - lo);
+      const KT* xj = cw < (uint32_t)span ? xs + 6 * cw : v + 6 * (int64_t)cj[k];
 #pragma unroll
       for (int c = 0; c < 6; ++c) xv[k][c] = xj[c];
     }
-    __builtin_amdgcn_sched_barrier(0);  // every load of the pass issued before the products
-    // a clamped round past the slice's end multiplies by exact zeros: the sum
-    // is unchanged, and no branch lets the loads sink below a test
-#pragma unroll
-    for (int k = 0; k < UR; ++k) {
-      const bool ok = qb + k < q1;
-#pragma unroll
-      for (int e = 0; e < 36; ++e) acc[e / 6] += ldv<KT>(av[k][e]) * (ok ? xv[k][e % 6] : KT(0));
-    }
+    sell6_accumulate<KT, VT, UR>(av, xv, qb, q1, acc);
   }
+  if (!b)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) bv[m] = KT(0);
+  const KT xb = x_base ? xb0 : KT(0);
 #pragma unroll
   for (int o = RPW; o < 64; o <<= 1)
 #pragma unroll
     for (int w = 0; w < 6; ++w) acc[w] += __shfl_xor(acc[w], o, 64);
-  if (u >= 6 || I < 0) return;
+  if (!live || u >= 6 || I < 0) return;
   KT r[6];
 #pragma unroll
   for (int m = 0; m < 6; ++m) r[m] = bv[m] - acc[m];
@@ -1012,28 +1090,29 @@ int upload_kt(Ctx* c, void** d, const std::vector<double>& h) {
 }
 
 // Rows per sliced-ELL slice of a coarse level.  Fewer rows per wave split a
-// row over more lanes (fewer dependent index -> gather rounds, more waves in
-// flight) at the price of more padding; these launches are bound by latency,
-// not bytes.  On the 64^3 cube 4 rows per wave gave the shortest Newton solve
-// (1 ms less than 8 over five alternating pairs; 2 was slower in FP64 mode).
-// ONSAS_SELL6_RPW (8, 4 or 2) overrides.
+// row over more lanes (fewer rounds per lane) at the price of more padding;
+// these launches are bound by latency, not bytes.  Measured on the 64^3 cube's
+// level 1 (14,182 rows, 39.7 blocks a row), batch-timed products: 8 rows per
+// wave 15.7 us (BF16, LDS window) and 33.1 us (FP64), 4 rows 21.3 / 34.3 us,
+// 2 rows 25.3 / 37.3 us.  ONSAS_SELL6_RPW (8, 4 or 2) overrides.
 int sell6_rows_per_wave() {
   if (const char* e = std::getenv("ONSAS_SELL6_RPW")) {
     const int r = std::atoi(e);
     if (r == 2 || r == 4 || r == 8) return r;
   }
-  return 4;
+  return 8;
 }
 
 // Rounds of a sliced-ELL coarse level whose loads a lane issues together
-// (k_sell6_cheb): 4 with BF16 values (128 VGPRs, 4 waves per SIMD), 2 with
-// FP64 values (36 doubles a round).  ONSAS_SELL6_ROUNDS (1, 2 or 4) overrides.
+// (k_sell6_cheb): 2 with BF16 values (109 VGPRs, no spill within the 16-wave
+// workgroup's 128), 1 with FP64 values (36 doubles a round); measured best at
+// 8 rows per wave.  ONSAS_SELL6_ROUNDS (1, 2 or 4) overrides.
 int sell6_rounds_in_flight(bool f64) {
   if (const char* e = std::getenv("ONSAS_SELL6_ROUNDS")) {
     const int r = std::atoi(e);
     if (r == 1 || r == 2 || r == 4) return r;
   }
-  return f64 ? 2 : 4;
+  return f64 ? 1 : 2;
 }
 
 }  // namespace
@@ -1151,6 +1230,10 @@ int amg_build(Ctx* c) {
       ONSAS_TRY(upload(c, &Lc.d_s6_col, S.s6_col));
       ONSAS_TRY(upload(c, &Lc.d_s6_src, S.s6_src));
       ONSAS_TRY(upload(c, &Lc.d_s6_row, S.s6_row));
+      if (!f64)  // the FP32 product reads x through an LDS window per workgroup
+        ONSAS_TRY(upload(c, &Lc.d_s6_win,
+                         host::sell6_windows(S.s6_ptr, S.s6_col, S.s6_src, na, kSell6Waves,
+                                             (int32_t)(kX6Lds / (6 * c->kt)))));
       ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(double) : sizeof(uint16_t)) * 36 * 64 *
                                         (size_t)Lc.nrounds6));
     }
@@ -1189,10 +1272,10 @@ template <typename KT, typename VT, int R, int UR>
 void launch_sell6_ru(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
                      double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
                      const PcgState* st, hipStream_t s) {
-  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R, UR>), dim3(nblocks(L.nslices6, 4)), dim3(256), 0, s,
-                     L.d_s6_ptr, L.d_s6_col, L.d_s6_row, (const VT*)A6, v, b, r_out,
-                     (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out, L.nslices6, L.n,
-                     st);
+  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R, UR>), dim3(nblocks(L.nslices6, sell6_waves<KT>())),
+                     dim3(64 * sell6_waves<KT>()), 0, s, L.d_s6_ptr, L.d_s6_col, L.d_s6_row, L.d_s6_win,
+                     (const VT*)A6, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out,
+                     x_base, x_out, L.nslices6, L.n, st);
 }
 
 template <typename KT, typename VT, int R>
@@ -1617,7 +1700,7 @@ void amg_free(Ctx* c) {
                     (void*)L.d_rt_ptr, (void*)L.d_rt_row, (void*)L.d_rt_pos, L.d_PT, L.d_W, (void*)L.d_wl_ptr, (void*)L.d_wl_k,
                     (void*)L.d_wl_p, (void*)L.d_gl_ptr, (void*)L.d_gl_p, (void*)L.d_gl_w,
                     (void*)L.d_s6_ptr, (void*)L.d_s6_col, (void*)L.d_s6_src,
-                    (void*)L.d_s6_row, L.d_A6s})
+                    (void*)L.d_s6_row, (void*)L.d_s6_win, L.d_A6s})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)A.d_dense, (void*)A.d_gjc, A.d_Ainv, (void*)A.d_part,
                   (void*)A.d_rzs})

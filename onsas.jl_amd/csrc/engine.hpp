@@ -133,6 +133,7 @@ struct AmgLevel {
   int rpw6 = 8;                       // rows per slice: 8, 4 or 2
   int ur6 = 4;                        // rounds whose loads a lane issues together
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
+  int32_t* d_s6_win = nullptr;        // per workgroup of 4 slices: its column window (lo, span)
   void* d_A6s = nullptr;              // the sliced-ELL values (BF16 mixed, FP64 FP64)
   double lam = 0;                     // lambda_max(D^-1 A), power iteration
   void* d_ev = nullptr;               // its last iterate (warm start, KT, bs n)

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <numeric>
@@ -238,6 +239,37 @@ struct SymbolicLevel {
   int rpw = 0;                                    // rows per slice of its sliced ELL (0: none)
   std::vector<int32_t> s6_ptr, s6_col, s6_src, s6_row;
 };
+
+// The column window of each group of `slices_per_group` consecutive slices of
+// a sliced-ELL level (one workgroup of the product kernel): lo (even) and span
+// (even unless it ends at node n, at most cap) such that [lo, lo + span) holds
+// the group's columns when they fit in cap nodes, and lo + span <= n.  Padding
+// slots (s6_src < 0) are ignored.  Returns lo and span interleaved per group.
+inline std::vector<int32_t> sell6_windows(const std::vector<int32_t>& s6_ptr,
+                                          const std::vector<int32_t>& s6_col,
+                                          const std::vector<int32_t>& s6_src, int64_t n,
+                                          int slices_per_group, int32_t cap) {
+  const int64_t nsl = (int64_t)s6_ptr.size() - 1;
+  const int64_t ng = (nsl + slices_per_group - 1) / slices_per_group;
+  std::vector<int32_t> w(2 * ng, 0);
+  for (int64_t g = 0; g < ng; ++g) {
+    int32_t lo = INT32_MAX, hi = -1;
+    const int64_t s0 = g * slices_per_group, s1 = std::min<int64_t>(nsl, s0 + slices_per_group);
+    for (int64_t q = 64 * (int64_t)s6_ptr[s0]; q < 64 * (int64_t)s6_ptr[s1]; ++q)
+      if (s6_src[q] >= 0) {
+        lo = std::min(lo, s6_col[q]);
+        hi = std::max(hi, s6_col[q]);
+      }
+    if (hi < 0) lo = hi = 0;
+    lo &= ~1;
+    int64_t span = std::min<int64_t>((int64_t)hi - lo + 1, cap);
+    span = (span + 1) & ~int64_t(1);
+    span = std::min<int64_t>(span, n - lo);
+    w[2 * g] = lo;
+    w[2 * g + 1] = (int32_t)span;
+  }
+  return w;
+}
 
 inline bool symbolic_level(const std::vector<int32_t>& rp, const std::vector<int32_t>& cl,
                            int64_t n, int bs, const std::vector<double>& Bcur,

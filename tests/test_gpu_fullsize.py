@@ -192,5 +192,8 @@ def test_identification_cube_48_single_parameter_fd():
               f"(of max |g| {errs[-1][1]:.2e})", flush=True)
     rel = np.array([x[0] for x in errs])
     print(f"  median rel {np.median(rel):.2e}, max rel {rel.max():.2e}", flush=True)
-    assert max(x[1] for x in errs) <= 1e-5
+    # every one of the 32 within 1e-5 of its own value (measured: median 4e-8,
+    # largest 7e-7, on the element with the smallest gradient)
+    assert rel.max() <= 1e-5
+    assert J0b == J0  # the FP64 re-solve is deterministic: no run-to-run noise in J
     sol64.analysis.state.close()
