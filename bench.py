@@ -96,6 +96,7 @@ def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False,
             sols.append(sol)
     tm = state.ctx.timings()
     nn, nnzb, ncol = state.ctx.sizes()
+    layout = state.ctx.gather_layout()
     levels = []
     if kernels and precond == "amg":
         # the levels' cycle products once more, back to back between two events
@@ -129,7 +130,7 @@ def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False,
         newton_iters=int(sum(sol.iterations())), pcg_iters=int(sol.linear_iterations()),
         max_rel_residual=float(max(r["dr_rel"] for r in sol.records)),
         criteria=sorted({type(r["criterion"]).__name__ for r in sol.records}),
-        disp_err_vs_closed_form=err, timings=tm, nn=nn, nnzb=nnzb, ncolors=ncol,
+        disp_err_vs_closed_form=err, timings=tm, nn=nn, nnzb=nnzb, ncolors=ncol, layout=layout,
         levels=levels, U_final=sol.U[-1])
     state.close()
     return out
@@ -143,6 +144,10 @@ def asm_rates(r, kt, ne):
       gather pass : contribution entries 16 x 4, state read 24 kt, forces 96
                     per tet; K 9 kt per block written once; row_ptr/col;
                     fixed mask, block-Jacobi 9 kt and f_int 24 per node.
+                    Staged (the default): the stored 16-bit entries (with
+                    padding) instead of 16 x 4, the element lists 4 per staged
+                    element, the step table 16 per step and 24 per workgroup,
+                    the diagonal index 4 per node; the states once.
     Beside it the reference algorithm's bytes (coloured BSR scatter: conn 16,
     grad N and V 104, u 96, slot map 64, K block and f_int read-modify-write:
     1,624 B/tet mixed, 2,776 B/tet FP64)."""
@@ -155,6 +160,11 @@ def asm_rates(r, kt, ne):
     b_elem = ne * (16 + 104 + 24 * kt + 96) + 24 * nn
     b_gather = (ne * (64 + 24 * kt + 96) + 9 * kt * nnzb + 4 * (nn + 1) + 4 * nnzb +
                 nn * (1 + 9 * kt + 24))
+    lay = r.get("layout") or {}
+    if tm["assembly_mode"] == 2:
+        b_gather = (ne * (24 * kt + 96) + 2 * lay["entries"] + 4 * lay["staged"] +
+                    16 * lay["steps"] + 24 * lay["groups"] + 9 * kt * nnzb + 4 * (nn + 1) +
+                    4 * nnzb + nn * (4 + 1 + 9 * kt + 24))
     tot = b_elem + b_gather
     if tm["assembly_mode"] == 0:
         tot = ne * (16 + 104 + 96 + 64 + 2 * 16 * 9 * kt + 2 * 12 * 8)
@@ -169,7 +179,8 @@ def asm_rates(r, kt, ne):
                 frac_design_bytes=gbs(tot, asm_ms) / HBM_PEAK_GBS,
                 elem_pass_frac=gbs(b_elem, elem_ms) / HBM_PEAK_GBS if elem_ms else None,
                 gather_pass_frac=gbs(b_gather, gather_ms) / HBM_PEAK_GBS if gather_ms else None,
-                mode="gather" if tm["assembly_mode"] == 1 else "scatter")
+                gather_pass_bytes=b_gather,
+                mode={2: "staged gather", 1: "gather"}.get(tm["assembly_mode"], "scatter"))
 
 
 def kernel_rooflines(r, kt):
@@ -401,7 +412,9 @@ def main():
                               round(asm["frac_reference_algorithm_bytes"], 4),
                               "frac_design_bytes": round(asm["frac_design_bytes"], 4),
                               "design_bytes_per_tet": round(asm["design_bytes_per_elem"], 1),
-                              "gather_pass_frac": round(asm["gather_pass_frac"] or 0, 4)},
+                              "gather_pass_frac": round(asm["gather_pass_frac"] or 0, 4),
+                              "gather_pass_ms": round(asm["gather_pass_ms"], 4),
+                              "mode": asm["mode"]},
     }
     fp = runs.get(mode_label("fp64", False, args.precond)) or bf
     if fp is not None:

@@ -119,7 +119,8 @@ typedef struct {
                                 PCG)                                                  */
     int64_t spmv_launches;   /* number of launches in spmv_ms                          */
     double elem_ms;          /* gather assembly: the per-element state pass alone     */
-    int64_t assembly_mode;   /* 1 gather (default), 0 coloured scatter                */
+    int64_t assembly_mode;   /* 1 gather (default), 2 staged gather (one tet group,  */
+                             /* ONSAS_GATHER=staged), 0 coloured scatter             */
     double cycle_spmv_ms;    /* multigrid: event-timed exact fine-level products of the
                                 V-cycle (r = b - K x, both, second iteration of each
                                 PCG)                                                  */
@@ -221,6 +222,13 @@ int onsas_set_U(onsas_ctx* ctx, const double* U); /* initial_state / resume     
 int onsas_get_fext(onsas_ctx* ctx, double* fext); /* external_forces(state)         */
 int onsas_get_fint(onsas_ctx* ctx, double* fint); /* internal_forces(state)         */
 int onsas_get_sizes(onsas_ctx* ctx, int64_t* nn, int64_t* nnzb, int64_t* ncolors);
+/* Layout of the gather assembly.  No counterpart in the reference (its
+ * assembly is a COO scatter); for the byte model of benchmarks and tests.
+ * out holds 6 values: [0] rows per workgroup of the staged gather (0: the
+ * list-driven gather or the scatter), [1] its workgroups, [2] staged elements
+ * over all workgroups, [3] contribution entries stored (with padding), [4] LDS
+ * bytes per workgroup, [5] steps. */
+int onsas_gather_layout(onsas_ctx* ctx, int64_t* out);
 
 /* Block-CSR pattern of the tangent (3x3 blocks, node indices): row_ptr nn+1,
  * col nnzb.  Replaces the SparseMatrixCSC pattern of tangent_matrix(state)

@@ -232,7 +232,7 @@ void account_assembly(Ctx* c) {
   c->tm.assemble_ms += c->last_assemble_ms;
   c->tm.scatter_ms += elapsed(c->ev[1], c->ev[2]);
   if (c->gather) c->tm.elem_ms += elapsed(c->ev[1], c->ev[8]);
-  c->tm.assembly_mode = c->gather ? 1 : 0;
+  c->tm.assembly_mode = c->gather ? (c->stage_rows > 0 ? 2 : 1) : 0;
   c->tm.n_assemble++;
 }
 
@@ -301,7 +301,9 @@ void onsas_destroy(onsas_ctx* ctx) {
                   c->d_pz, c->d_pp0, c->d_pp1, c->d_pq, c->d_part, c->d_sc, c->d_face_conn,
                   c->d_face_grp, c->d_nf_ptr, c->d_nf_idx, c->d_pressures, c->d_gm_V, c->d_gm_w,
                   c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds,
-                  c->d_contrib, c->d_sell_groups};
+                  c->d_contrib, c->d_sell_groups, (void*)c->d_stage_groups,
+                  (void*)c->d_stage_steps, c->d_stage, c->d_stage_words, c->d_stage_pairs,
+                  c->d_pair_row, c->d_colmask};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -603,6 +605,19 @@ int onsas_get_fint(onsas_ctx* ctx, double* f) {
   ONSAS_TRY(check_ready(c));
   if (!f) return set_err(c, ONSAS_ERR_ARG, "NULL output");
   ONSAS_TRY(download_state(c, f, c->d_fint));
+  return ONSAS_OK;
+}
+
+int onsas_gather_layout(onsas_ctx* ctx, int64_t* out) {
+  Ctx* c = C(ctx);
+  ONSAS_TRY(check_ready(c));
+  if (!out) return set_err(c, ONSAS_ERR_ARG, "null output");
+  out[0] = c->stage_rows;
+  out[1] = c->stage_rows ? c->nstage_groups : 0;
+  out[2] = c->stage_rows ? c->stage_total : 0;
+  out[3] = c->gather ? c->ncontrib : 0;
+  out[4] = c->stage_rows ? (int64_t)onsas::stage_gather_lds(c) : 0;
+  out[5] = c->stage_rows ? c->stage_nsteps : 0;
   return ONSAS_OK;
 }
 

@@ -568,6 +568,44 @@ __device__ inline void add_contrib(const Contrib<KT, HM>& c, KT acc[9], double& 
   f2 += c.f[2];
 }
 
+// The end of a gathered block: fixed dofs zero their rows and columns (unit
+// diagonal entry, as the scatter path does), the block is stored at o (value v
+// at o[64 v]), and on the diagonal the FP64-computed inverse of the finished
+// block (block-Jacobi) and the row's f_int are written.
+template <typename KT>
+__device__ inline void gather_finish(KT acc[9], uint32_t mr, uint32_t mc, bool dg, KT* __restrict__ o,
+                                     KT* __restrict__ Dinv, double* __restrict__ fint, int64_t r,
+                                     double f0, double f1, double f2) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (((mr >> i) & 1u) | ((mc >> j) & 1u)) acc[i * 3 + j] = (dg && i == j) ? KT(1) : KT(0);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) o[64 * v] = acc[v];
+  if (dg) {
+    double A[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[q] = (double)acc[q];
+    const double c0 = A[4] * A[8] - A[5] * A[7], cf1 = A[5] * A[6] - A[3] * A[8],
+                 cf2 = A[3] * A[7] - A[4] * A[6];
+    const double rdet = 1.0 / (A[0] * c0 + A[1] * cf1 + A[2] * cf2);
+    KT* di = Dinv + 9 * r;
+    di[0] = (KT)(c0 * rdet);
+    di[1] = (KT)((A[2] * A[7] - A[1] * A[8]) * rdet);
+    di[2] = (KT)((A[1] * A[5] - A[2] * A[4]) * rdet);
+    di[3] = (KT)(cf1 * rdet);
+    di[4] = (KT)((A[0] * A[8] - A[2] * A[6]) * rdet);
+    di[5] = (KT)((A[2] * A[3] - A[0] * A[5]) * rdet);
+    di[6] = (KT)(cf2 * rdet);
+    di[7] = (KT)((A[1] * A[6] - A[0] * A[7]) * rdet);
+    di[8] = (KT)((A[0] * A[4] - A[1] * A[3]) * rdet);
+    fint[3 * r] = f0;
+    fint[3 * r + 1] = f1;
+    fint[3 * r + 2] = f2;
+  }
+}
+
 // One wave per sliced-ELL group (64 rows at one round), one lane per block.
 // The contribution rounds are uniform over the wave (padded lists), the
 // element-group lookup is a predicated select (no dynamic indexing of the
@@ -620,35 +658,213 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs ga, const SellGroup* 
     add_contrib(c0, acc, f0, f1, f2);
     add_contrib(c1, acc, f0, f1, f2);
   }
-  const uint32_t mr = mask[r], mc = mask[cc];
+  gather_finish(acc, mask[r], mask[cc], dg, Ks + 9 * (int64_t)sg.base + lane, Dinv, fint, r, f0, f1,
+                f2);
+}
+
+// ---- staged gather ---------------------------------------------------------------
+// One workgroup per `rows` consecutive rows of a 64-row slice (the lists of
+// host_setup.hpp::staged_gather).  First the states of the workgroup's elements
+// go to LDS, element i of its ascending list at column i (SP values apart);
+// the loads run along the list, which on a structured mesh is a few runs of
+// consecutive elements, so each plane's load is coalesced.  Then lane l of a
+// step sums its block (row, round) from LDS through 16-bit entries, in the
+// lists' element order -- the sums and order of k_gather -- and ends it as
+// k_gather does; the diagonal's lane reads the nodal forces from HBM.  The
+// next step's entries and column are loaded before this step's sums.
+constexpr int kStageWords = 12;  // entry words of a step held in registers (24 contributions)
+
+template <typename KT, bool HM, int SP>
+__device__ inline void stage_add(const KT* __restrict__ ls, uint32_t en, KT acc[9]) {
+  const bool valid = en != host::kNoStage;
+  const KT* x = ls + (valid ? (int)(en >> 4) : 0) * SP;
+  const int a = (en >> 2) & 3, b = en & 3;
+  const KT c1 = valid ? x[0] : KT(0), c2 = valid ? x[1] : KT(0);
+  KT va[3], vb[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    va[i] = x[2 + 3 * a + i];
+    vb[i] = x[2 + 3 * b + i];
+  }
+  const int p = a <= b ? pair_index(a, b) : pair_index(b, a);
+  const KT g = valid ? x[14 + p] : KT(0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      if (((mr >> i) & 1u) | ((mc >> j) & 1u)) acc[i * 3 + j] = (dg && i == j) ? KT(1) : KT(0);
-  KT* o = Ks + 9 * (int64_t)sg.base + lane;
+    for (int j = 0; j < 3; ++j) acc[i * 3 + j] += c1 * va[i] * vb[j] + c2 * vb[i] * va[j];
+  acc[0] += g;
+  acc[4] += g;
+  acc[8] += g;
+  if (HM) {
+    const KT h = valid ? x[24 + p] : KT(0);
+    KT M[6];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) o[64 * v] = acc[v];
-  if (dg) {
-    double A[9];
+    for (int q = 0; q < 6; ++q) M[q] = x[34 + q];
+    acc[0] += h * M[0];
+    acc[1] += h * M[1];
+    acc[2] += h * M[2];
+    acc[3] += h * M[1];
+    acc[4] += h * M[3];
+    acc[5] += h * M[4];
+    acc[6] += h * M[2];
+    acc[7] += h * M[4];
+    acc[8] += h * M[5];
+  }
+}
+
+// KS: list entries per thread in the staging (elements and incidences alike),
+// all loads issued before the first store
+template <typename KT, int NST, int NT, int KS>
+__global__ __launch_bounds__(NT) void k_gather_lds(
+    const KT* __restrict__ st, const double* __restrict__ fe, int64_t ne,
+    const host::StageGroup* __restrict__ groups, int64_t ngroups, const int32_t* __restrict__ stage,
+    const host::StageStep* __restrict__ steps, const uint32_t* __restrict__ words,
+    const uint32_t* __restrict__ pairs, const int32_t* __restrict__ pair_row, int rlog2,
+    int cap, int pairs_cap, int steps_cap, const int32_t* __restrict__ row_ptr,
+    const uint8_t* __restrict__ colmask, const int32_t* __restrict__ diag,
+    const uint8_t* __restrict__ mask, const int32_t* __restrict__ sell_ptr, int64_t rstride,
+    KT* __restrict__ Ks, KT* __restrict__ Dinv, double* __restrict__ fint, int64_t nn) {
+  constexpr bool HM = NST == kTetStateM;
+  constexpr int SP = NST + 1;  // odd stride: the staging stores hit distinct banks
+  constexpr int NW = NT / 64;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  host::StageStep* lst = reinterpret_cast<host::StageStep*>(lds_raw);
+  double* lfe = reinterpret_cast<double*>(lst + steps_cap);  // 3 per incidence
+  KT* ls = reinterpret_cast<KT*>(lfe + 3 * (size_t)pairs_cap);
+  (void)cap;
+  const int64_t nb = gridDim.x, bid = blockIdx.x;
+  const int64_t L = (bid & 7) * (nb >> 3) + (bid >> 3);  // XCD-aware, as k_gather
+  if (L >= ngroups) return;  // the whole workgroup, before its barrier
+  const host::StageGroup g = groups[L];
+  // workgroup L holds rows L << rlog2 on: the row loads need no group record
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row0 = L << rlog2;
+  const int64_t r = row0 + (lane & ((1 << rlog2) - 1));
+  const bool live = r < nn;
+  const int32_t rs = live ? row_ptr[r] : 0;
+  const int32_t len = live ? row_ptr[r + 1] - rs : 0;
+  const int32_t dround = live ? diag[r] - rs : -1;
+  const uint32_t mr = live ? mask[r] : 0u;
+  const int32_t pr = live ? pair_row[r] : 0;
+  const int64_t sbase = sell_ptr[row0 >> 6];
+  KT* const orow = Ks + (r & 63);
+
+  // this wave's first step (table entry, entry words, column mask), in flight
+  // during the staging
+  host::StageStep cur{0, 0, 0};
+  uint32_t wc[kStageWords];
+  uint32_t mc = 0;
+  if (wave < g.nsteps) {
+    cur = steps[g.step0 + wave];
+    const int m = cur.m0 + (lane >> rlog2);
+    if (m < len) mc = colmask[rs + m];
+  }
 #pragma unroll
-    for (int q = 0; q < 9; ++q) A[q] = (double)acc[q];
-    const double c0 = A[4] * A[8] - A[5] * A[7], cf1 = A[5] * A[6] - A[3] * A[8],
-                 cf2 = A[3] * A[7] - A[4] * A[6];
-    const double rdet = 1.0 / (A[0] * c0 + A[1] * cf1 + A[2] * cf2);
-    KT* di = Dinv + 9 * r;
-    di[0] = (KT)(c0 * rdet);
-    di[1] = (KT)((A[2] * A[7] - A[1] * A[8]) * rdet);
-    di[2] = (KT)((A[1] * A[5] - A[2] * A[4]) * rdet);
-    di[3] = (KT)(cf1 * rdet);
-    di[4] = (KT)((A[0] * A[8] - A[2] * A[6]) * rdet);
-    di[5] = (KT)((A[2] * A[3] - A[0] * A[5]) * rdet);
-    di[6] = (KT)(cf2 * rdet);
-    di[7] = (KT)((A[1] * A[6] - A[0] * A[7]) * rdet);
-    di[8] = (KT)((A[0] * A[4] - A[1] * A[3]) * rdet);
-    fint[3 * r] = f0;
-    fint[3 * r + 1] = f1;
-    fint[3 * r + 2] = f2;
+  for (int k = 0; k < kStageWords; ++k)
+    wc[k] = 2 * k < cur.qwidth ? words[cur.cbase + 64 * k + lane] : 0xffffffffu;
+
+  // staging: the step table, the element states and the rows' nodal forces
+  for (int i = threadIdx.x; i < g.nsteps; i += NT) lst[i] = steps[g.step0 + i];
+  int32_t ei[KS];
+  uint32_t pi[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = (int)threadIdx.x + k * NT;
+    ei[k] = i < g.nstage ? stage[g.stage_off + i] : -1;
+    pi[k] = i < g.npairs ? pairs[g.pair_off + i] : 0xffffffffu;
+  }
+  KT v[KS][NST];
+  double fv[KS][3];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    if (ei[k] >= 0) {
+#pragma unroll
+      for (int p = 0; p < NST; ++p) v[k][p] = st[p * ne + ei[k]];
+    }
+    if (pi[k] != 0xffffffffu) {
+      const int64_t e = pi[k] & 0x0fffffffu;
+      const int a = (int)(pi[k] >> 28);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fv[k][d] = fe[(3 * a + d) * ne + e];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int i = (int)threadIdx.x + k * NT;
+    if (ei[k] >= 0) {
+#pragma unroll
+      for (int p = 0; p < NST; ++p) ls[i * SP + p] = v[k][p];
+    }
+    if (pi[k] != 0xffffffffu) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lfe[3 * i + d] = fv[k][d];
+    }
+  }
+  __syncthreads();
+
+  for (int t = wave; t < g.nsteps; t += NW) {
+    // the next step's loads first
+    const int tn = t + NW;
+    host::StageStep nx{0, 0, 0};
+    uint32_t wn[kStageWords];
+    uint32_t mn = 0;
+    if (tn < g.nsteps) {
+      nx = lst[tn];
+      const int m2 = nx.m0 + (lane >> rlog2);
+      if (m2 < len) mn = colmask[rs + m2];
+    }
+#pragma unroll
+    for (int k = 0; k < kStageWords; ++k)
+      wn[k] = 2 * k < nx.qwidth ? words[nx.cbase + 64 * k + lane] : 0xffffffffu;
+
+    const int m = cur.m0 + (lane >> rlog2);
+    if (m < len) {
+      const bool dg = m == dround;
+      const int nw = cur.qwidth >> 1;
+      KT acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = KT(0);
+#pragma unroll
+      for (int k = 0; k < kStageWords; ++k)
+        if (k < nw) {  // uniform over the wave
+          stage_add<KT, HM, SP>(ls, wc[k] & 0xffffu, acc);
+          stage_add<KT, HM, SP>(ls, wc[k] >> 16, acc);
+        }
+      for (int k = kStageWords; k < nw; ++k) {  // longer lists (unstructured meshes)
+        const uint32_t w = words[cur.cbase + 64 * k + lane];
+        stage_add<KT, HM, SP>(ls, w & 0xffffu, acc);
+        stage_add<KT, HM, SP>(ls, w >> 16, acc);
+      }
+      // f_int: the row's incidences in its diagonal block's contribution order
+      double f0 = 0, f1 = 0, f2 = 0;
+      if (dg) {  // valid entries are a prefix of the list
+#pragma unroll
+        for (int k = 0; k < kStageWords; ++k)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            if (k < nw && ((wc[k] >> (16 * h)) & 0xffffu) != host::kNoStage) {
+              const double* x = lfe + 3 * (pr + 2 * k + h);
+              f0 += x[0];
+              f1 += x[1];
+              f2 += x[2];
+            }
+        for (int k = kStageWords; k < nw; ++k) {
+          const uint32_t w = words[cur.cbase + 64 * k + lane];
+          for (int h = 0; h < 2; ++h)
+            if (((w >> (16 * h)) & 0xffffu) != host::kNoStage) {
+              const double* x = lfe + 3 * (pr + 2 * k + h);
+              f0 += x[0];
+              f1 += x[1];
+              f2 += x[2];
+            }
+        }
+      }
+      gather_finish(acc, mr, mc, dg, orow + 9 * (sbase + rstride * m), Dinv, fint, r, f0, f1, f2);
+    }
+    cur = nx;
+    mc = mn;
+#pragma unroll
+    for (int k = 0; k < kStageWords; ++k) wc[k] = wn[k];
   }
 }
 
@@ -1071,6 +1287,56 @@ int assemble_t(Ctx* c) {
   return ONSAS_OK;
 }
 
+// LDS of one staged-gather workgroup: the step table, the rows' nodal forces
+// and the staged states
+size_t stage_lds_bytes(const Ctx* c) {
+  const int nst = c->groups[0].hasM ? kTetStateM : kTetStateBase;
+  return sizeof(host::StageStep) * (size_t)c->stage_steps_cap +
+         3 * sizeof(double) * (size_t)c->stage_pairs_cap +
+         (size_t)c->kt * (nst + 1) * (size_t)c->stage_cap;
+}
+
+// set_attr: raise the kernel's dynamic LDS limit (setup); else launch it
+template <typename KT, int NST, int KS>
+int stage_run(Ctx* c, bool set_attr) {
+  const size_t lds = stage_lds_bytes(c);
+  auto kern = k_gather_lds<KT, NST, kStageThreads, KS>;
+  if (set_attr) {
+    ONSAS_HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return ONSAS_OK;
+  }
+  const ElemGroup& g = c->groups[0];
+  const unsigned gb = (unsigned)((c->nstage_groups + 7) / 8 * 8);  // a multiple of 8 (XCD mapping)
+  int rlog2 = 0;
+  while ((1 << rlog2) < c->stage_rows) ++rlog2;
+  hipLaunchKernelGGL(kern, dim3(gb), dim3(kStageThreads), lds, c->stream, (const KT*)g.d_state,
+                     g.d_fe, g.ne, c->d_stage_groups, c->nstage_groups, c->d_stage,
+                     c->d_stage_steps, c->d_stage_words, c->d_stage_pairs, c->d_pair_row, rlog2,
+                     c->stage_cap, c->stage_pairs_cap, c->stage_steps_cap, c->d_row_ptr,
+                     c->d_colmask, c->d_diag, c->d_fixmask, c->d_sell_ptr, c->sell_rstride,
+                     (KT*)c->d_Ksell, (KT*)c->d_Dinv, c->d_fint, c->nn);
+  ONSAS_HIP_TRY(c, hipGetLastError());
+  return ONSAS_OK;
+}
+
+template <typename KT, int NST>
+int stage_dispatch_ks(Ctx* c, bool set_attr) {
+  switch (c->stage_ks) {
+    case 1: return stage_run<KT, NST, 1>(c, set_attr);
+    case 2: return stage_run<KT, NST, 2>(c, set_attr);
+    case 3: return stage_run<KT, NST, 3>(c, set_attr);
+    case 4: return stage_run<KT, NST, 4>(c, set_attr);
+  }
+  return set_err(c, ONSAS_ERR_UNSUPPORTED, "staged gather: entries per thread out of range");
+}
+
+template <typename KT>
+int stage_dispatch(Ctx* c, bool set_attr) {
+  return c->groups[0].hasM ? stage_dispatch_ks<KT, kTetStateM>(c, set_attr)
+                           : stage_dispatch_ks<KT, kTetStateBase>(c, set_attr);
+}
+
 template <typename KT>
 int assemble_gather_t(Ctx* c) {
   hipStream_t s = c->stream;
@@ -1089,6 +1355,12 @@ int assemble_gather_t(Ctx* c) {
   }
   ONSAS_HIP_TRY(c, hipGetLastError());
   ONSAS_HIP_TRY(c, hipEventRecord(c->ev[8], s));
+  if (c->stage_rows > 0) {
+    ONSAS_TRY(stage_dispatch<KT>(c, false));
+    ONSAS_HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    c->bsr_stale = true;
+    return ONSAS_OK;
+  }
   const int64_t ngrp = (int64_t)c->sell_groups.size();
   const unsigned gb = (grid_for(ngrp, 4) + 7) / 8 * 8;  // a multiple of 8 (XCD mapping)
   bool hm = false;
@@ -1115,6 +1387,13 @@ int launch_assemble(Ctx* c) {
                                                 : assemble_gather_t<float>(c);
   return c->precision == ONSAS_PRECISION_FP64 ? assemble_t<double>(c) : assemble_t<float>(c);
 }
+
+int stage_gather_prepare(Ctx* c) {
+  return c->precision == ONSAS_PRECISION_FP64 ? stage_dispatch<double>(c, true)
+                                              : stage_dispatch<float>(c, true);
+}
+
+size_t stage_gather_lds(const Ctx* c) { return stage_lds_bytes(c); }
 
 int launch_gather_init(Ctx* c) {
   const unsigned grid = grid_for(c->nn, 256);

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/onsas_hip.h"
+#include "host_setup.hpp"
 
 namespace onsas {
 
@@ -94,6 +95,10 @@ struct ElemGroup {
 // 00 01 02 03 11 12 13 22 23 33.
 constexpr int kTetStateBase = 24, kTetStateM = 40, kTrussState = 5;
 constexpr int kMaxGatherGroups = 8;
+// staged gather: threads per workgroup, and the most list entries a thread
+// stages (the kernel is instantiated for 1 .. kStageMaxEntries)
+constexpr int kStageThreads = 256;
+constexpr int kStageMaxEntries = 4;
 
 // One sliced-ELL group = the 64 lanes of one slice at one round: the unit of
 // work (one wave) of the gather assembly.
@@ -254,6 +259,25 @@ struct Ctx {
   SellGroup* d_sell_groups = nullptr;
   uint32_t* d_contrib = nullptr;  // (global element << 4) | (a << 2) | b, 0xffffffff = none
   int64_t ncontrib = 0;           // stored entries including padding
+  // staged gather (one tet group): per workgroup, the states of the elements
+  // of `stage_rows` consecutive rows are copied into LDS and summed from there
+  // (host_setup.hpp::staged_gather); stage_rows = 0 selects the lists above
+  int stage_rows = 0;
+  int stage_cap = 0;        // LDS columns per workgroup (the largest element list)
+  int stage_pairs_cap = 0;  // the most incidences of a workgroup's rows
+  int stage_steps_cap = 0;  // the most steps of a workgroup
+  int stage_ks = 0;         // list entries per thread in the staging
+  int64_t sell_rstride = 0; // sliced-ELL blocks between the rounds of a slice
+  int64_t nstage_groups = 0;
+  int64_t stage_total = 0;   // staged elements over all workgroups
+  int64_t stage_nsteps = 0;
+  host::StageGroup* d_stage_groups = nullptr;
+  host::StageStep* d_stage_steps = nullptr;
+  int32_t* d_stage = nullptr;
+  uint32_t* d_stage_words = nullptr;
+  uint32_t* d_stage_pairs = nullptr;
+  int32_t* d_pair_row = nullptr;
+  uint8_t* d_colmask = nullptr;  // per block: the fixed-dof mask of its column
   bool bsr_stale = false;         // d_K does not hold the last assembly yet
 
   // device
@@ -350,6 +374,8 @@ int launch_element_outputs(Ctx* c, int g, const double* U, double* f, double* K,
                            double* strain);
 int launch_fext(Ctx* c);                                        // nodal + pressure faces
 int launch_adjoint_shear(Ctx* c, int g, const double* lambda_dev, double* dJdG_dev);
+int stage_gather_prepare(Ctx* c);     // the staged gather's LDS attribute
+size_t stage_gather_lds(const Ctx* c);  // its LDS bytes per workgroup
 int launch_gather_init(Ctx* c);                                 // zero padding, identity Dinv
 int ensure_bsr(Ctx* c);                                         // d_K from d_Ksell if stale
 int launch_tangent_apply_f64(Ctx* c, const double* v_dev, double* y_dev);  // y = K(U) v, FP64

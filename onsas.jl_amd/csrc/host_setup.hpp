@@ -411,5 +411,167 @@ inline bool symbolic_level(const std::vector<int32_t>& rp, const std::vector<int
   return true;
 }
 
+// ---- staged gather assembly ------------------------------------------------------------
+
+// Work of one workgroup of the staged gather: `rows` consecutive tangent rows
+// of one 64-row slice; workgroup g holds rows g * rows + [0, rows).  Its
+// elements (every element with a node among the rows) are listed ascending in
+// stage[stage_off, stage_off + nstage); the workgroup copies their state
+// planes into LDS, column i for list entry i.  Its steps are
+// steps[step0, step0 + nsteps).  pairs[pair_off, pair_off + npairs) are the
+// (element, local node) incidences of its rows, row after row, each row's in
+// the order of its diagonal block's contributions (e | a << 28): their nodal
+// forces go to LDS too, row r's from pair_row[r] on.
+struct StageGroup {
+  int32_t row0, nstage, step0, nsteps;
+  int64_t stage_off;
+  int64_t pair_off;
+  int32_t npairs, pad;
+};
+// One step of a wave: lane l on row row0 + l % rows at round m0 + l / rows.
+// Its contribution entries are the words cbase + 64 k + l, k < qwidth / 2,
+// two 16-bit entries (j << 4 | a << 2 | b, j the LDS column) per word, entry
+// 2k in the low half; kNoStage pads.  qwidth is even.
+struct StageStep {
+  int64_t cbase;
+  int32_t qwidth, m0;
+};
+constexpr uint16_t kNoStage = 0xffff;
+constexpr int kMaxStage = 4095;  // LDS columns per workgroup (12-bit entry field)
+
+struct StagedGather {
+  int rows = 0;
+  int max_stage = 0;
+  int max_pairs = 0;
+  std::vector<StageGroup> groups;
+  std::vector<StageStep> steps;
+  std::vector<int32_t> stage;
+  std::vector<uint32_t> words;
+  std::vector<uint32_t> pairs;
+  std::vector<int32_t> pair_row;  // per row: its first pair, relative to its workgroup's
+};
+
+// Builds the staged gather of one element group with nnode = 4 over the
+// tangent pattern (row_ptr; block (r, k) is round k of row r in the sliced-ELL
+// layout, slices of 64 rows with slice_rounds[s] rounds).  slot[(a*4+b)*ne + e]
+// is the block that element e's (a, b) block adds to; diag[r] the block (r, r)
+// (-1: none).  rows divides 64.  The contributions of a block are listed in
+// ascending element order, as the list-driven gather sums them.  Returns false
+// (G untouched but `max_stage`) when a workgroup would stage more than cap
+// elements.
+inline bool staged_gather(int64_t nn, const std::vector<int32_t>& row_ptr,
+                          const std::vector<int32_t>& diag, const int32_t* slot, int64_t ne,
+                          const std::vector<int32_t>& slice_rounds, int rows, int cap,
+                          StagedGather& G) {
+  const int64_t nnzb = row_ptr[nn];
+  cap = std::min(cap, kMaxStage);
+  // contributions per block, ascending element order
+  std::vector<int64_t> cptr(nnzb + 1, 0);
+  for (int64_t k = 0; k < 16 * ne; ++k) cptr[slot[k] + 1]++;
+  for (int64_t b = 0; b < nnzb; ++b) cptr[b + 1] += cptr[b];
+  std::vector<uint32_t> cent(cptr[nnzb]);
+  {
+    std::vector<int64_t> pos(cptr.begin(), cptr.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+      for (int ab = 0; ab < 16; ++ab) cent[pos[slot[(int64_t)ab * ne + e]]++] = (uint32_t)((e << 4) | ab);
+  }
+  const int64_t nslices = (int64_t)slice_rounds.size();
+  const int per = 64 / rows;
+  std::vector<int32_t> mark(ne, -1), loc(ne, 0);
+  std::vector<int32_t> list;
+  // first pass: the element lists (and the largest)
+  StagedGather S;
+  S.rows = rows;
+  for (int64_t s = 0; s < nslices; ++s)
+    for (int h = 0; h < per; ++h) {
+      const int64_t r0 = 64 * s + (int64_t)h * rows;
+      if (r0 >= nn) break;
+      const int32_t gid = (int32_t)S.groups.size();
+      list.clear();
+      for (int64_t r = r0; r < std::min<int64_t>(nn, r0 + rows); ++r)
+        for (int64_t q = cptr[row_ptr[r]]; q < cptr[row_ptr[r + 1]]; ++q) {
+          const int64_t e = cent[q] >> 4;
+          if (mark[e] != gid) {
+            mark[e] = gid;
+            list.push_back((int32_t)e);
+          }
+        }
+      std::sort(list.begin(), list.end());
+      S.max_stage = std::max(S.max_stage, (int)list.size());
+      StageGroup g;
+      g.row0 = (int32_t)r0;
+      g.nstage = (int32_t)list.size();
+      g.stage_off = (int64_t)S.stage.size();
+      g.step0 = g.nsteps = 0;
+      g.pair_off = 0;
+      g.npairs = g.pad = 0;
+      S.groups.push_back(g);
+      S.stage.insert(S.stage.end(), list.begin(), list.end());
+    }
+  G.max_stage = S.max_stage;
+  if (S.max_stage > cap) return false;
+  // second pass: steps and entries
+  struct Lane {
+    int64_t blk;
+    int32_t cnt;
+  };
+  std::vector<StageStep> wsteps;
+  std::vector<std::vector<Lane>> wlanes;
+  S.pair_row.assign(nn, 0);
+  for (StageGroup& g : S.groups) {
+    for (int32_t i = 0; i < g.nstage; ++i) loc[S.stage[g.stage_off + i]] = i;
+    // the rows' incidences: the contributions of their diagonal blocks
+    g.pair_off = (int64_t)S.pairs.size();
+    for (int64_t r = g.row0; r < std::min<int64_t>(nn, g.row0 + (int64_t)rows); ++r) {
+      S.pair_row[r] = (int32_t)(S.pairs.size() - g.pair_off);
+      if (diag[r] < 0) continue;
+      for (int64_t q = cptr[diag[r]]; q < cptr[diag[r] + 1]; ++q)
+        S.pairs.push_back((uint32_t)((cent[q] >> 4) | (uint64_t)((cent[q] >> 2) & 3u) << 28));
+    }
+    g.npairs = (int32_t)(S.pairs.size() - g.pair_off);
+    S.max_pairs = std::max(S.max_pairs, g.npairs);
+    const int64_t s = g.row0 / 64;
+    const int nst = (slice_rounds[s] + per - 1) / per;
+    wsteps.assign(nst, StageStep{0, 0, 0});
+    wlanes.assign(nst, std::vector<Lane>(64, Lane{-1, 0}));
+    for (int t = 0; t < nst; ++t) {
+      wsteps[t].m0 = t * per;
+      for (int l = 0; l < 64; ++l) {
+        const int64_t r = g.row0 + l % rows;
+        const int m = t * per + l / rows;
+        if (r >= nn || m >= row_ptr[r + 1] - row_ptr[r]) continue;
+        const int64_t blk = row_ptr[r] + m;
+        wlanes[t][l] = Lane{blk, (int32_t)(cptr[blk + 1] - cptr[blk])};
+        wsteps[t].qwidth = std::max(wsteps[t].qwidth, wlanes[t][l].cnt);
+      }
+      wsteps[t].qwidth += wsteps[t].qwidth & 1;
+    }
+    // heaviest steps first: wave w takes steps w, w + waves, ... in this order
+    std::vector<int> ord(nst);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int x, int y) { return wsteps[x].qwidth > wsteps[y].qwidth; });
+    g.step0 = (int32_t)S.steps.size();
+    g.nsteps = nst;
+    for (int t : ord) {
+      StageStep st = wsteps[t];
+      st.cbase = (int64_t)S.words.size();
+      S.steps.push_back(st);
+      S.words.resize(S.words.size() + 64 * (size_t)(st.qwidth / 2), 0xffffffffu);
+      for (int l = 0; l < 64; ++l) {
+        const Lane& ln = wlanes[t][l];
+        for (int32_t q = 0; q < ln.cnt; ++q) {
+          const uint32_t en = cent[cptr[ln.blk] + q];
+          const uint32_t v = ((uint32_t)loc[en >> 4] << 4) | (en & 15u);
+          uint32_t& w = S.words[st.cbase + 64 * (q / 2) + l];
+          w = (q & 1) ? ((w & 0xffffu) | (v << 16)) : ((w & 0xffff0000u) | v);
+        }
+      }
+    }
+  }
+  G = std::move(S);
+  return true;
+}
+
 }  // namespace host
 }  // namespace onsas

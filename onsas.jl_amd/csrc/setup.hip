@@ -94,6 +94,75 @@ std::vector<int32_t> gather_order(const ElemGroup& g) {
   return host::gather_order({g.conn.data(), g.nnode, g.ne});
 }
 
+// Staged gather, chosen by ONSAS_GATHER=staged for a model of one tet group:
+// the largest row count per workgroup (64, 32, 16 or 8) whose largest element
+// list fits the LDS budget (ONSAS_GATHER_LDS_KB, default 64 KiB) and the
+// registers of the staging, or the count ONSAS_GATHER_ROWS names.  The
+// list-driven gather is the default: on the 64^3 cube it is faster (each
+// staged workgroup waits on about three dependent HBM round trips, with
+// 16 waves per CU at most).
+int build_staged_gather(Ctx* c, const std::vector<std::vector<int32_t>>& slots) {
+  c->stage_rows = 0;
+  const char* gm = std::getenv("ONSAS_GATHER");
+  if (kSellLanes != 1 || c->groups.size() != 1 || c->groups[0].type != kTet ||
+      !(gm && std::string(gm) == "staged"))
+    return ONSAS_OK;
+  const ElemGroup& g = c->groups[0];
+  const char* kb = std::getenv("ONSAS_GATHER_LDS_KB");
+  int lds_max = 0;
+  ONSAS_HIP_TRY(c, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                         c->device));
+  const int64_t budget =
+      std::min<int64_t>(lds_max, 1024 * (int64_t)(kb ? std::max(8, std::atoi(kb)) : 64));
+  const char* fr = std::getenv("ONSAS_GATHER_ROWS");
+  const int forced = fr ? std::atoi(fr) : 0;
+  const int64_t per_col = (int64_t)c->kt * (g.nstate + 1);
+  // list entries per thread: each holds a state column and 3 forces in registers
+  const int ks_max = std::max<int>(1, std::min<int>(kStageMaxEntries,
+                                                    128 / (g.nstate * (int)c->kt / 4 + 6)));
+  host::StagedGather S;
+  for (int rows : {64, 32, 16, 8}) {
+    if (forced && rows != forced) continue;
+    int steps_cap = 0;
+    for (int32_t w : c->slice_rounds) steps_cap = std::max(steps_cap, (w + 64 / rows - 1) / (64 / rows));
+    const int64_t cap = std::min<int64_t>(
+        {(budget - 16 * (int64_t)steps_cap) / per_col, (int64_t)ks_max * kStageThreads,
+         host::kMaxStage});
+    if (cap < 1) continue;
+    host::StagedGather T;
+    if (!host::staged_gather(c->nn, c->row_ptr, c->diag, slots[0].data(), g.ne, c->slice_rounds,
+                             rows, (int)cap, T))
+      continue;
+    const int64_t lds = 16 * (int64_t)steps_cap + 24 * (int64_t)T.max_pairs +
+                        per_col * std::max(1, T.max_stage);
+    const int ks = (std::max({T.max_stage, T.max_pairs, 1}) + kStageThreads - 1) /
+                   kStageThreads;
+    if (lds > budget || ks > ks_max) continue;
+    c->stage_steps_cap = steps_cap;
+    c->stage_ks = ks;
+    S = std::move(T);
+    break;
+  }
+  if (S.rows == 0) return ONSAS_OK;  // nothing fits: the list-driven gather
+  c->stage_rows = S.rows;
+  c->stage_cap = std::max(1, S.max_stage);
+  c->stage_pairs_cap = std::max(1, S.max_pairs);
+  c->nstage_groups = (int64_t)S.groups.size();
+  c->stage_total = (int64_t)S.stage.size();
+  c->stage_nsteps = (int64_t)S.steps.size();
+  std::vector<uint8_t> cm(c->nnzb);
+  for (int64_t k = 0; k < c->nnzb; ++k) cm[k] = c->fixmask[c->col[k]];
+  ONSAS_TRY(dev_upload(c, &c->d_stage_groups, S.groups.data(), S.groups.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_stage_steps, S.steps.data(), S.steps.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_stage, S.stage.data(), S.stage.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_stage_words, S.words.data(), S.words.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_stage_pairs, S.pairs.data(), std::max<size_t>(S.pairs.size(), 1)));
+  ONSAS_TRY(dev_upload(c, &c->d_pair_row, S.pair_row.data(), S.pair_row.size()));
+  ONSAS_TRY(dev_upload(c, &c->d_colmask, cm.data(), cm.size()));
+  c->ncontrib = 2 * (int64_t)S.words.size();
+  return stage_gather_prepare(c);
+}
+
 // Contribution lists of the gather assembly.  Block (r, k) of the tangent
 // receives the element blocks (e, a, b) whose slot is that block; they are
 // listed by (global element, a, b), so every assembly sums them in the same
@@ -103,6 +172,21 @@ std::vector<int32_t> gather_order(const ElemGroup& g) {
 // 0xffffffff.  Also allocates the per-element state planes.
 int build_gather(Ctx* c, const std::vector<std::vector<int32_t>>& slots, int64_t rstride) {
   const int64_t nnzb = c->nnzb, nn = c->nn;
+  for (ElemGroup& g : c->groups) {
+    if (g.type == kTet) {
+      g.hasM = (g.mat == ONSAS_MAT_SVK);
+      g.nstate = g.hasM ? kTetStateM : kTetStateBase;
+    } else {
+      g.hasM = false;
+      g.nstate = kTrussState;
+    }
+    ONSAS_TRY(dev_alloc(c, &g.d_state, c->kt * g.nstate * (size_t)g.ne));
+    ONSAS_TRY(dev_alloc(c, (void**)&g.d_fe,
+                        sizeof(double) * (g.type == kTet ? 12 : 3) * (size_t)g.ne));
+  }
+  c->sell_rstride = rstride;
+  ONSAS_TRY(build_staged_gather(c, slots));
+  if (c->stage_rows > 0) return ONSAS_OK;
   std::vector<int64_t> ptr(nnzb + 1, 0);
   for (size_t gi = 0; gi < c->groups.size(); ++gi)
     for (int32_t blk : slots[gi]) ptr[blk + 1]++;
@@ -163,18 +247,6 @@ int build_gather(Ctx* c, const std::vector<std::vector<int32_t>>& slots, int64_t
   }
   ONSAS_TRY(dev_upload(c, &c->d_contrib, lists.data(), lists.size()));
   ONSAS_TRY(dev_upload(c, &c->d_sell_groups, c->sell_groups.data(), c->sell_groups.size()));
-  for (ElemGroup& g : c->groups) {
-    if (g.type == kTet) {
-      g.hasM = (g.mat == ONSAS_MAT_SVK);
-      g.nstate = g.hasM ? kTetStateM : kTetStateBase;
-    } else {
-      g.hasM = false;
-      g.nstate = kTrussState;
-    }
-    ONSAS_TRY(dev_alloc(c, &g.d_state, c->kt * g.nstate * (size_t)g.ne));
-    ONSAS_TRY(dev_alloc(c, (void**)&g.d_fe,
-                        sizeof(double) * (g.type == kTet ? 12 : 3) * (size_t)g.ne));
-  }
   return ONSAS_OK;
 }
 

@@ -33,7 +33,8 @@ EXPORTED = (
     "onsas_set_fixed_dofs", "onsas_finalize", "onsas_set_linear_solver", "onsas_set_loads",
     "onsas_assemble", "onsas_newton_step", "onsas_get_U", "onsas_snapshot_U",
     "onsas_read_snapshot", "onsas_set_U", "onsas_get_fext",
-    "onsas_get_fint", "onsas_get_sizes", "onsas_get_pattern", "onsas_get_tangent",
+    "onsas_get_fint", "onsas_get_sizes", "onsas_gather_layout", "onsas_get_pattern",
+    "onsas_get_tangent",
     "onsas_tangent_matvec", "onsas_tangent_matvec_noalloc", "onsas_element_forces",
     "onsas_get_stress_strain",
     "onsas_linear_solve", "onsas_adjoint_shear", "onsas_set_shear_moduli", "onsas_locate_points",
@@ -118,6 +119,7 @@ def load():
         "onsas_get_fext": ([_vp, _vp], C.c_int),
         "onsas_get_fint": ([_vp, _vp], C.c_int),
         "onsas_get_sizes": ([_vp, _i64p, _i64p, _i64p], C.c_int),
+        "onsas_gather_layout": ([_vp, _i64p], C.c_int),
         "onsas_get_pattern": ([_vp, _vp, _vp], C.c_int),
         "onsas_get_tangent": ([_vp, _vp], C.c_int),
         "onsas_tangent_matvec": ([_vp, _vp, _vp], C.c_int),
@@ -302,6 +304,14 @@ class Context:
         nn, nnzb, nc = C.c_int64(), C.c_int64(), C.c_int64()
         self._check(self.L.onsas_get_sizes(self.h, C.byref(nn), C.byref(nnzb), C.byref(nc)))
         return nn.value, nnzb.value, nc.value
+
+    def gather_layout(self):
+        """Staged-gather layout: rows per workgroup (0: not staged), workgroups,
+        staged elements, stored contribution entries, LDS bytes per workgroup,
+        steps."""
+        out = (C.c_int64 * 6)()
+        self._check(self.L.onsas_gather_layout(self.h, out))
+        return dict(zip(("rows", "groups", "staged", "entries", "lds_bytes", "steps"), list(out)))
 
     def pattern(self):
         nn, nnzb, _ = self.sizes()

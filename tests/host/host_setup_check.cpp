@@ -87,6 +87,92 @@ Mesh scrambled(int n, unsigned seed) {
   return m;
 }
 
+// staged gather of the tets: every (row, round) of the sliced-ELL layout is one
+// lane of one step of its row's workgroup, and that lane's entries decode to
+// the block's contributions in ascending element order
+void check_staged(const Mesh& m, const Pattern& P) {
+  const int64_t ne = (int64_t)m.tets.size() / 4, nn = m.nn;
+  std::vector<int32_t> slot(16 * ne);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        slot[(a * 4 + b) * ne + e] = slot_of(P, m.tets[4 * e + a], m.tets[4 * e + b]);
+  const int64_t nslices = (nn + 63) / 64;
+  std::vector<int32_t> rounds(nslices, 0);
+  for (int64_t r = 0; r < nn; ++r)
+    rounds[r / 64] = std::max(rounds[r / 64], P.row_ptr[r + 1] - P.row_ptr[r]);
+  // expected contributions per block, ascending element
+  std::vector<std::vector<uint32_t>> want(P.col.size());
+  for (int64_t e = 0; e < ne; ++e)
+    for (int ab = 0; ab < 16; ++ab) want[slot[ab * ne + e]].push_back((uint32_t)((e << 4) | ab));
+  for (int rows : {64, 32, 16, 8}) {
+    StagedGather G;
+    CHECK(!staged_gather(nn, P.row_ptr, P.diag, slot.data(), ne, rounds, rows, 0, G) || ne == 0);
+    CHECK(staged_gather(nn, P.row_ptr, P.diag, slot.data(), ne, rounds, rows, kMaxStage, G));
+    CHECK(G.rows == rows);
+    std::vector<int> seen(P.col.size(), 0);
+    for (const StageGroup& g : G.groups) {
+      CHECK(g.row0 % rows == 0 && g.row0 < nn && g.nstage <= G.max_stage);
+      const int32_t* sl = &G.stage[g.stage_off];
+      for (int32_t i = 1; i < g.nstage; ++i) CHECK(sl[i - 1] < sl[i]);
+      // the listed elements are exactly those with a node among the rows
+      std::vector<int32_t> need;
+      for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 4; ++a) {
+          const int32_t r = m.tets[4 * e + a];
+          if (r >= g.row0 && r < g.row0 + rows) {
+            need.push_back((int32_t)e);
+            break;
+          }
+        }
+      CHECK((int32_t)need.size() == g.nstage && std::equal(need.begin(), need.end(), sl));
+      for (int32_t t = 0; t < g.nsteps; ++t) {
+        const StageStep& st = G.steps[g.step0 + t];
+        CHECK(st.qwidth % 2 == 0 && st.m0 % (64 / rows) == 0);
+        CHECK(st.cbase + 32 * (int64_t)st.qwidth <= (int64_t)G.words.size());
+        for (int l = 0; l < 64; ++l) {
+          const int64_t r = g.row0 + l % rows;
+          const int mm = st.m0 + l / rows;
+          const bool has = r < nn && mm < P.row_ptr[r + 1] - P.row_ptr[r];
+          const std::vector<uint32_t>* w = has ? &want[P.row_ptr[r] + mm] : nullptr;
+          if (has) seen[P.row_ptr[r] + mm]++;
+          for (int q = 0; q < st.qwidth; ++q) {
+            const uint32_t word = G.words[st.cbase + 64 * (q / 2) + l];
+            const uint32_t en = (q & 1) ? word >> 16 : word & 0xffffu;
+            if (!has || q >= (int)w->size()) {
+              CHECK(en == kNoStage);
+              continue;
+            }
+            const uint32_t j = en >> 4;
+            CHECK((int32_t)j < g.nstage);
+            CHECK((((uint32_t)sl[j]) << 4 | (en & 15u)) == (*w)[q]);
+          }
+        }
+      }
+    }
+    for (size_t b = 0; b < P.col.size(); ++b) CHECK(seen[b] == 1);
+    // workgroup g holds rows g * rows on; its incidences are its rows' diagonal
+    // contributions, row after row, from pair_row[r] on
+    CHECK((int64_t)G.pair_row.size() == nn);
+    for (size_t gi = 0; gi < G.groups.size(); ++gi) {
+      const StageGroup& g = G.groups[gi];
+      CHECK(g.row0 == (int64_t)gi * rows && g.npairs <= G.max_pairs);
+      int32_t off = 0;
+      for (int64_t r = g.row0; r < std::min<int64_t>(nn, g.row0 + rows); ++r) {
+        CHECK(G.pair_row[r] == off);
+        if (P.diag[r] < 0) continue;
+        for (uint32_t w : want[P.diag[r]]) {
+          const uint32_t pr = G.pairs[g.pair_off + off++];
+          CHECK((pr & 0x0fffffffu) == (w >> 4) && (pr >> 28) == ((w >> 2) & 3u));
+          CHECK(m.tets[4 * (w >> 4) + ((w >> 2) & 3u)] == r);
+        }
+      }
+      CHECK(off == g.npairs);
+    }
+    CHECK(G.pairs.size() == (size_t)(4 * ne));
+  }
+}
+
 void check_mesh(const Mesh& m, const char* name) {
   std::vector<ConnView> views = {{m.tets.data(), 4, (int64_t)m.tets.size() / 4}};
   if (!m.trusses.empty()) views.push_back({m.trusses.data(), 2, (int64_t)m.trusses.size() / 2});
@@ -199,6 +285,7 @@ void check_mesh(const Mesh& m, const char* name) {
     bs = 6;
     B.swap(S.Bc);
   }
+  check_staged(m, P);
   std::printf("ok %s (nodes %lld, blocks %zu)\n", name, (long long)m.nn, P.col.size());
 }
 

@@ -27,17 +27,22 @@ MATS = {"svk": (_capi.MAT_SVK, (0.6e6, 0.4e6)), "nh": (_capi.MAT_NEOHOOKEAN, (1.
 
 @contextlib.contextmanager
 def _assembly_mode(mode):
-    """ONSAS_ASSEMBLY is read when a context is finalized."""
-    old = os.environ.get("ONSAS_ASSEMBLY")
-    if mode is not None:
+    """ONSAS_ASSEMBLY and ONSAS_GATHER are read when a context is finalized;
+    mode "staged" is the gather assembly with its states staged in LDS."""
+    old = {k: os.environ.get(k) for k in ("ONSAS_ASSEMBLY", "ONSAS_GATHER")}
+    if mode == "staged":
+        os.environ.pop("ONSAS_ASSEMBLY", None)
+        os.environ["ONSAS_GATHER"] = "staged"
+    elif mode is not None:
         os.environ["ONSAS_ASSEMBLY"] = mode
     try:
         yield
     finally:
-        if old is None:
-            os.environ.pop("ONSAS_ASSEMBLY", None)
-        else:
-            os.environ["ONSAS_ASSEMBLY"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def _ctx(prec, coords, tets=None, mat="svk", trusses=None, strain=0, E=1.0, A=None, fixed=(),
@@ -131,12 +136,13 @@ def _bsr_vs_oracle(Kg, Ko, fixed, n, tol):
     assert np.abs(Kg[np.ix_(free, fixed)]).max(initial=0) == 0
 
 
-@pytest.mark.parametrize("assembly", ["gather", "scatter"])
+@pytest.mark.parametrize("assembly", ["gather", "staged", "scatter"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("mat", ["svk", "nh", "linear"])
 def test_assembly_matches_oracle(oracle, prec, mat, assembly):
-    """Gather (per-block contribution lists) and coloured-scatter assembly of
-    f_int and K vs the oracle's COO->CSR assembly."""
+    """Gather (per-block contribution lists), staged gather (element states in
+    LDS per workgroup of rows) and coloured-scatter assembly of f_int and K vs
+    the oracle's COO->CSR assembly."""
     print(f"[gpu] assembly {mat} {prec} {assembly}", flush=True)
     s, sm = workloads.cube_extension(n=4, jitter=0.1, seed=3)
     fixed = s.fixed_dofs
@@ -159,7 +165,7 @@ def test_assembly_matches_oracle(oracle, prec, mat, assembly):
     yo = out["K"] @ xf
     yo[fixed] = x[fixed]
     np.testing.assert_allclose(y, yo, rtol=0, atol=(1e-11 if prec == "fp64" else 3e-6) * np.abs(yo).max())
-    assert c.timings()["assembly_mode"] == (1 if assembly == "gather" else 0)
+    assert c.timings()["assembly_mode"] == {"gather": 1, "staged": 2, "scatter": 0}[assembly]
     c.close()
 
 
@@ -208,7 +214,7 @@ def test_assembly_mixed_groups_vs_oracle(oracle, prec, assembly):
     c.close()
 
 
-@pytest.mark.parametrize("assembly", ["gather", "scatter"])
+@pytest.mark.parametrize("assembly", ["gather", "staged", "scatter"])
 def test_assembly_is_deterministic(assembly):
     """Two assemblies of the same state give the same bits (no float atomics)."""
     print(f"[gpu] assembly determinism {assembly}", flush=True)
