@@ -189,10 +189,11 @@ def kernel_rooflines(r, kt):
       k_sell_cheb (fine level, 2 launches per iteration with the degree-1 fine
         smoother): nnzb (9 kt + 4) + 4.5 n kt (x gathered, b, r / x, b,
         block-Jacobi 3, z: 3 and 6 values per dof, mean 4.5)
-      k_sell6_cheb (level 1, sliced ELL, 4 launches per iteration):
-        nnzb_1 (36 vb + 4) + 49.5 kt per row (vb = 2, BF16 copy, in mixed mode;
-        8 in FP64 mode; per row x gathered 6, b 6, D^-1 36, outputs 6 or 12:
-        60, 18, 60 and 60 values for the four products)."""
+      k_sell6_cheb (level 1, sliced ELL, 4 launches per iteration with the
+        degree-2 smoother): nnzb_1 (36 vb + 4) + 49.5 kt per row (vb = 2, BF16
+        copy, in mixed mode; 8 in FP64 mode; per row x gathered 6, b 6, D^-1
+        36, outputs 6 or 12: 60, 18, 60 and 60 values for the four products);
+        with the degree-1 smoother 2 launches, 18 and 54 values per row."""
     out = {}
     lv = {e["level"]: e for e in r.get("levels", [])}
     if 0 in lv and "batch_ms_per_launch" in lv[0]:
@@ -208,15 +209,17 @@ def kernel_rooflines(r, kt):
     if 1 in lv and "batch_ms_per_launch" in lv[1]:
         e = lv[1]
         vb = 2 if kt == 4 else 8
-        b = int(e["nnzb"] * (36 * vb + 4) + 49.5 * kt * e["n"])
+        deg1 = os.environ.get("ONSAS_AMG_COARSE_DEGREE", "2") == "1"
+        b = int(e["nnzb"] * (36 * vb + 4) + (36.0 if deg1 else 49.5) * kt * e["n"])
         ms = e["batch_ms_per_launch"]
         rpw = os.environ.get("ONSAS_SELL6_RPW", "8")
         ur = os.environ.get("ONSAS_SELL6_ROUNDS", "2" if kt == 4 else "1")
+        wv = os.environ.get("ONSAS_SELL6_WAVES", "8") if kt == 4 else "4"
         out["k_sell6_cheb"] = dict(
             kernel=f"k_sell6_cheb<{'float, unsigned short' if kt == 4 else 'double, double'}, "
-                   f"{rpw}, {ur}>",
+                   f"{rpw}, {ur}, {wv}>",
             what="level-1 sliced-ELL 6x6 operator product + Chebyshev epilogue",
-            bytes_per_launch=b, ms_per_launch=ms, launches_per_pcg_iter=4,
+            bytes_per_launch=b, ms_per_launch=ms, launches_per_pcg_iter=2 if deg1 else 4,
             GBps=b / (ms * 1e-3) / 1e9)
     for k in out.values():
         k["ms_per_pcg_iter"] = k["ms_per_launch"] * k["launches_per_pcg_iter"]

@@ -613,7 +613,7 @@ struct Sell6Layout {
 };
 
 // Large levels >= 1 on their sliced-ELL copy: one wave per slice of RPW rows,
-// 64 / RPW lanes per row (lane l serves row l % RPW), kSell6Waves slices per
+// 64 / RPW lanes per row (lane l serves row l % RPW), WV slices per
 // workgroup.  The workgroup first copies the x values of its column window
 // (host-computed: lo, span <= kX6Lds bytes) into LDS with coalesced 16-byte
 // loads, issued with the first pass's column indices, values and row ids, so a
@@ -626,13 +626,11 @@ struct Sell6Layout {
 // and used after the products.  After the reduction over the lanes of a row,
 // lanes l / RPW < 6 finish that component of row s6_row[RPW s + l % RPW]
 // (rows sorted by length within windows; -1: none).
-// Waves per workgroup: 16 with the LDS window (FP32 x), 4 without it (FP64 x:
-// its 36 values a round need the registers that the window's copy would take).
-constexpr int kSell6Waves = 16;
+// WV waves (slices) per workgroup: 4, 8 or 16 with the LDS window (FP32 x), 4
+// without it (FP64 x: its 36 values a round need the registers that the
+// window's copy would take).
 template <typename KT>
 constexpr bool sell6_window() { return sizeof(KT) == 4; }
-template <typename KT>
-constexpr int sell6_waves() { return sell6_window<KT>() ? kSell6Waves : 4; }
 template <typename KT, typename VT, int RPW, int UR>
 __device__ inline void sell6_load_pass(const int32_t* __restrict__ s6_col, const VT* __restrict__ A6,
                                        int32_t qb, int32_t qlast, int l, int32_t (&cj)[UR],
@@ -671,8 +669,8 @@ __device__ inline void sell6_accumulate(const VT (&av)[UR][36], const KT (&xv)[U
   }
 }
 
-template <typename KT, typename VT, int RPW, int UR>
-__global__ __launch_bounds__(64 * sell6_waves<KT>()) void k_sell6_cheb(
+template <typename KT, typename VT, int RPW, int UR, int WV>
+__global__ __launch_bounds__(64 * WV) void k_sell6_cheb(
     const int32_t* __restrict__ s6_ptr, const int32_t* __restrict__ s6_col,
     const int32_t* __restrict__ s6_row, const int32_t* __restrict__ s6_win,
     const VT* __restrict__ A6, const KT* __restrict__ v, const KT* b, KT* r_out,
@@ -681,12 +679,12 @@ __global__ __launch_bounds__(64 * sell6_waves<KT>()) void k_sell6_cheb(
   static_assert(RPW == 2 || RPW == 4 || RPW == 8, "6 components need >= 8 lanes per row");
   static_assert(UR >= 1 && UR <= 4, "1 to 4 rounds in flight (registers: 36 values each)");
   constexpr bool kWin = sell6_window<KT>();
-  constexpr int kThreads = 64 * sell6_waves<KT>();
+  constexpr int kThreads = 64 * WV;
   constexpr int kWinUnits = kWin ? (int)(kX6Lds / 16) : 1;  // 16-byte units of the window
   constexpr int kPerThread = (kWinUnits + kThreads - 1) / kThreads;
   __shared__ __align__(16) uint4 xs4[kWinUnits];
   const KT* xs = (const KT*)xs4;
-  const int64_t s0 = (int64_t)blockIdx.x * sell6_waves<KT>() + (threadIdx.x >> 6);
+  const int64_t s0 = (int64_t)blockIdx.x * WV + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   // every wave of the workgroup reaches the barrier: a wave past the last slice
   // reads that slice's header, runs its rounds as zeros and writes nothing
@@ -1104,15 +1102,31 @@ int sell6_rows_per_wave() {
 }
 
 // Rounds of a sliced-ELL coarse level whose loads a lane issues together
-// (k_sell6_cheb): 2 with BF16 values (109 VGPRs, no spill within the 16-wave
-// workgroup's 128), 1 with FP64 values (36 doubles a round); measured best at
-// 8 rows per wave.  ONSAS_SELL6_ROUNDS (1, 2 or 4) overrides.
+// (k_sell6_cheb): 2 with BF16 values, 1 with FP64 values (36 doubles a round);
+// measured best at 8 rows per wave (BF16, 8 waves: 13.6 us at 2 rounds, 15.9 us
+// at 1, 13.9 us at 4).  ONSAS_SELL6_ROUNDS (1, 2 or 4) overrides.
 int sell6_rounds_in_flight(bool f64) {
   if (const char* e = std::getenv("ONSAS_SELL6_ROUNDS")) {
     const int r = std::atoi(e);
     if (r == 1 || r == 2 || r == 4) return r;
   }
   return f64 ? 1 : 2;
+}
+
+// Waves (slices) per workgroup of a sliced-ELL coarse level's product: each
+// FP32 workgroup copies its window of x into LDS, so larger groups share one
+// copy; smaller groups spread a small level over more CUs.  Level 1 of the
+// 64^3 cube (1,773 slices of 8 rows), batch-timed, 8 rows per wave, 2 rounds:
+// 16 waves 15.8 us (111 workgroups: fewer than the 256 CUs), 8 waves 13.6 us,
+// 4 waves 14.1 us (a smaller window, more gathers from global memory).  FP64:
+// 4 (no window).  ONSAS_SELL6_WAVES (4, 8 or 16) overrides in FP32.
+int sell6_waves_per_group(bool f64) {
+  if (f64) return 4;
+  if (const char* e = std::getenv("ONSAS_SELL6_WAVES")) {
+    const int w = std::atoi(e);
+    if (w == 4 || w == 8 || w == 16) return w;
+  }
+  return 8;
 }
 
 }  // namespace
@@ -1130,6 +1144,10 @@ int amg_build(Ctx* c) {
     // ONSAS_AMG_FINE_DEGREE=2 selects the degree-2 Chebyshev smoother there
     const char* e = std::getenv("ONSAS_AMG_FINE_DEGREE");
     A.fine_degree = (e && std::atoi(e) == 2) ? 2 : 1;
+    // the coarser levels' smoothing: degree 2 (four products of a level per
+    // cycle) or, with ONSAS_AMG_COARSE_DEGREE=1, the damped-Jacobi step (two)
+    const char* ec = std::getenv("ONSAS_AMG_COARSE_DEGREE");
+    A.coarse_degree = (ec && std::atoi(ec) == 1) ? 1 : 2;
   }
   const int64_t nn = c->nn;
   const bool f64 = c->precision == ONSAS_PRECISION_FP64;
@@ -1192,6 +1210,7 @@ int amg_build(Ctx* c) {
     if (S.rpw > 0) {
       Lc.rpw6 = S.rpw;
       Lc.ur6 = sell6_rounds_in_flight(f64);
+      Lc.waves6 = sell6_waves_per_group(f64);
       Lc.nslices6 = (int64_t)S.s6_ptr.size() - 1;
       Lc.nrounds6 = S.s6_ptr.back();
     }
@@ -1232,7 +1251,7 @@ int amg_build(Ctx* c) {
       ONSAS_TRY(upload(c, &Lc.d_s6_row, S.s6_row));
       if (!f64)  // the FP32 product reads x through an LDS window per workgroup
         ONSAS_TRY(upload(c, &Lc.d_s6_win,
-                         host::sell6_windows(S.s6_ptr, S.s6_col, S.s6_src, na, kSell6Waves,
+                         host::sell6_windows(S.s6_ptr, S.s6_col, S.s6_src, na, Lc.waves6,
                                              (int32_t)(kX6Lds / (6 * c->kt)))));
       ONSAS_TRY(alloc(c, &Lc.d_A6s, (f64 ? sizeof(double) : sizeof(uint16_t)) * 36 * 64 *
                                         (size_t)Lc.nrounds6));
@@ -1268,14 +1287,29 @@ int amg_build(Ctx* c) {
 
 namespace {
 
+template <typename KT, typename VT, int R, int UR, int WV>
+void launch_sell6_ruw(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
+                      double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
+                      const PcgState* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R, UR, WV>), dim3(nblocks(L.nslices6, WV)),
+                     dim3(64 * WV), 0, s, L.d_s6_ptr, L.d_s6_col, L.d_s6_row, L.d_s6_win,
+                     (const VT*)A6, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out,
+                     x_base, x_out, L.nslices6, L.n, st);
+}
+
 template <typename KT, typename VT, int R, int UR>
 void launch_sell6_ru(const AmgLevel& L, const void* A6, const KT* v, const KT* b, KT* r_out,
                      double c1, double c2, KT* d_out, const KT* x_base, KT* x_out,
                      const PcgState* st, hipStream_t s) {
-  hipLaunchKernelGGL((k_sell6_cheb<KT, VT, R, UR>), dim3(nblocks(L.nslices6, sell6_waves<KT>())),
-                     dim3(64 * sell6_waves<KT>()), 0, s, L.d_s6_ptr, L.d_s6_col, L.d_s6_row, L.d_s6_win,
-                     (const VT*)A6, v, b, r_out, (const KT*)L.d_Dinv, (KT)c1, (KT)c2, d_out,
-                     x_base, x_out, L.nslices6, L.n, st);
+  if constexpr (sell6_window<KT>()) {
+    if (L.waves6 == 16)
+      return launch_sell6_ruw<KT, VT, R, UR, 16>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out,
+                                                 st, s);
+    if (L.waves6 == 8)
+      return launch_sell6_ruw<KT, VT, R, UR, 8>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out,
+                                                st, s);
+  }
+  launch_sell6_ruw<KT, VT, R, UR, 4>(L, A6, v, b, r_out, c1, c2, d_out, x_base, x_out, st, s);
 }
 
 template <typename KT, typename VT, int R>
@@ -1464,13 +1498,14 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     hipLaunchKernelGGL((k_jacobi<KT, 3>), dim3(nblocks(3 * L.n, 256)), dim3(256), 0, s,
                        (const KT*)c->d_Dinv, b0, (KT*)L.d_d0, L.n, (KT)cheb(L.lam).inv_theta, st);
   }
+  auto degree1 = [&](int l) { return (l == 0 ? A.fine_degree : A.coarse_degree) == 1; };
   // down: pre-smoothing, residual, restriction
   for (int l = 0; l + 1 < nl; ++l) {
     AmgLevel& L = A.lev[l];
     const Cheb ch = cheb(L.lam);
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
     KT* r = (KT*)L.d_r;
-    if (l == 0 && A.fine_degree == 1) {
+    if (degree1(l)) {
       // degree 1: x = d0 itself (prolonged into in place), the exact r = b - A d0
       op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st,
                   evs(l, 0), evs(l, 1));
@@ -1500,11 +1535,14 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     const AmgLevel& C = A.lev[l + 1];
     const Cheb ch = cheb(L.lam);
     const KT* b = l == 0 ? b0 : (const KT*)L.d_b;
-    const bool deg1 = l == 0 && A.fine_degree == 1;
+    const bool deg1 = degree1(l);
     KT* d0 = (KT*)L.d_d0;
     KT* x = deg1 ? d0 : (KT*)L.d_x;
     KT* r = (KT*)L.d_r;
-    KT* z = l == 0 ? z0 : x;
+    // the level's result goes to x (coarser levels) for the next prolongation;
+    // the degree-1 step reads its input at neighbouring rows, so it writes
+    // there out of place
+    KT* z = l == 0 ? z0 : (KT*)L.d_x;
     if (l == 0)
       hipLaunchKernelGGL((k_sa_prolong_node<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
                          dim3(nblocks(L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
@@ -1553,8 +1591,8 @@ namespace {
 // The cycle's products of level l, in cycle order, on the last solve's vectors
 // (no stop flag), between two events.  Level 0 alternates its two products
 // (degree 1: r = b - A d0 and z = d0 + D^-1 (b - A d0) / theta); a sliced-ELL
-// level >= 1 cycles through its four (x = d0 + d1, r = b - A x, r and d0 =
-// D^-1 r / theta, x += d0 + d1).
+// level >= 1 the same two with the degree-1 smoother, else its four (x = d0 +
+// d1, r = b - A x, r and d0 = D^-1 r / theta, x += d0 + d1).
 template <typename KT>
 int amg_time_level_products_t(Ctx* c, int l, int launches, double* ms) {
   Amg& A = c->amg;
@@ -1578,6 +1616,12 @@ int amg_time_level_products_t(Ctx* c, int l, int launches, double* ms) {
         else
           op_cheb<KT>(c, 0, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr);
       }
+    } else if (A.coarse_degree == 1) {
+      const KT* b = (const KT*)L.d_b;
+      if (i & 1)
+        op_cheb<KT>(c, l, d0, b, nullptr, 0.0, ch.inv_theta, nullptr, nullptr, x, nullptr);
+      else
+        op_cheb<KT>(c, l, d0, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, nullptr);
     } else {
       const KT* b = (const KT*)L.d_b;
       switch (i & 3) {

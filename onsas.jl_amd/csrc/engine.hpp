@@ -137,6 +137,7 @@ struct AmgLevel {
   int64_t nslices6 = 0, nrounds6 = 0;
   int rpw6 = 8;                       // rows per slice: 8, 4 or 2
   int ur6 = 4;                        // rounds whose loads a lane issues together
+  int waves6 = 4;                     // slices (waves) per workgroup of the product
   int32_t *d_s6_ptr = nullptr, *d_s6_col = nullptr, *d_s6_src = nullptr, *d_s6_row = nullptr;
   int32_t* d_s6_win = nullptr;        // per workgroup of 4 slices: its column window (lo, span)
   void* d_A6s = nullptr;              // the sliced-ELL values (BF16 mixed, FP64 FP64)
@@ -172,6 +173,7 @@ struct Amg {
   bool stale = true;       // the coarse levels belong to an earlier tangent
   bool step_stale = true;  // ... to an earlier load step
   int fine_degree = 1;     // Chebyshev degree of the finest level's smoothing (1 or 2)
+  int coarse_degree = 2;   // and of the coarser levels'
   std::vector<AmgLevel> lev;
   int nc = 0;                         // dofs of the coarsest level (dense inverse)
   double* d_dense = nullptr;          // nc x nc FP64 work (the inverse in place)

@@ -752,6 +752,39 @@ def test_amg_fine_degree_two(oracle, prec, monkeypatch):
     assert its["2"] <= its["1"]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_amg_coarse_degree_one(prec, monkeypatch):
+    """The damped-Jacobi smoothing on the coarser levels (ONSAS_AMG_COARSE_DEGREE
+    =1, read when the hierarchy is built) on the 40^3 cube, whose level 1 has
+    the sliced-ELL operator: the degree-1 step reads its input at neighbouring
+    rows and writes out of place.  Both degrees meet the tolerance in the true
+    residual against the engine's tangent; degree 2 needs no more iterations."""
+    print(f"[gpu] AMG coarse degree 1 {prec}", flush=True)
+    s, sm, p = workloads.cube_compression(n=40, jitter=0.02, seed=2)
+    fixed = np.asarray(s.fixed_dofs)
+    a = workloads.nh_uniaxial_alpha(-p * 0.5, 8.3e5, 3.8e5)
+    b = workloads.nh_uniaxial_beta(a, 8.3e5, 3.8e5)
+    U = workloads.homogeneous_displacement(sm.coords, a, b)
+    rtol = 1e-9 if prec == "fp64" else 1e-6
+    rhs = np.random.default_rng(4).normal(size=3 * sm.nn)
+    rhs[fixed] = 0
+    its = {}
+    for deg in ("1", "2"):
+        monkeypatch.setenv("ONSAS_AMG_COARSE_DEGREE", deg)
+        c = _ctx(prec, sm.coords, sm.tets, "nh", fixed=fixed)
+        c.set_linear_solver(_capi.LINSOLVE_PCG, rtol=rtol, maxiter=500, precond=_capi.PRECOND_AMG)
+        c.set_U(U)
+        c.assemble()
+        x, its[deg], _ = c.linear_solve(rhs)
+        K = c.tangent_bsr()
+        res = np.linalg.norm(rhs - K @ x) / np.linalg.norm(rhs)
+        assert res <= (3 * rtol if prec == "fp64" else 2e-5), (deg, res)
+        assert c.amg_levels() >= 3
+        c.close()
+    print(f"  iterations degree 1 {its['1']}, degree 2 {its['2']}", flush=True)
+    assert its["2"] <= its["1"] <= 80
+
+
 @pytest.mark.parametrize("pc", ["block-jacobi", "amg"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_warm_start_matches_cold(prec, pc, monkeypatch):
