@@ -58,6 +58,45 @@ inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); 
 
 __device__ inline bool pcg_stopped(const PcgState* st) { return st && (st->done | st->breakdown); }
 
+// The PCG's stop check at the start of the cycle's first kernel (256 threads),
+// run by every workgroup alike: ||r||^2 from the update's np partials, summed in
+// the order of the 512-wide sum the next k_pcg_A applies to them (virtual
+// thread t' adds entries t', t' + 512, ... in order, then the butterfly over
+// its wave, then the 8 wave sums in order; here threads t and t + 256 are one
+// thread's two accumulators), and the same rule: stop once rr <= tol2 or k
+// reached maxiter.  Workgroup 0 records the stop.  True: the PCG has stopped.
+__device__ inline bool pcg_stop_check(const double* __restrict__ prr, int np, PcgState* st,
+                                      double* sh) {
+  if (st->done | st->breakdown) return true;  // the same for every thread
+  const int t = threadIdx.x;
+  double a = 0, b = 0;
+  for (int i = t; i < np; i += 512) a += prr[i];
+  for (int i = t + 256; i < np; i += 512) b += prr[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((t & 63) == 0) {
+    sh[t >> 6] = a;
+    sh[4 + (t >> 6)] = b;
+  }
+  __syncthreads();
+  double rr = 0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) rr += sh[w];
+  const int k = st->k;  // iterations completed
+  const bool conv = rr <= st->tol2;
+  const bool stop = conv || k >= st->maxiter;
+  if (stop && blockIdx.x == 0 && t == 0) {
+    st->done = 1;
+    st->iters = k;
+    st->converged = conv ? 1 : 0;
+    st->rr_last = rr;
+  }
+  return stop;
+}
+
 template <typename T>
 __device__ inline T pick6(const T (&a)[6], int u) {
   T r = a[0];
@@ -465,9 +504,14 @@ __global__ __launch_bounds__(256) void k_sell_cheb(
     const int32_t* __restrict__ scol, const VT* __restrict__ sval, const KT* __restrict__ v,
     const KT* b, KT* r_out, const KT* __restrict__ Dinv, KT c1, KT c2, KT* __restrict__ d_out,
     const KT* x_base, KT* x_out, int64_t nslices, int64_t nn, const PcgState* st,
-    double* __restrict__ prz) {
+    double* __restrict__ prz, const double* __restrict__ chk_rr, int chk_np,
+    PcgState* __restrict__ chk_st) {
   const int64_t s0 = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  if (chk_rr) {  // the whole workgroup, before any wave leaves
+    __shared__ double csh[8];
+    if (pcg_stop_check(chk_rr, chk_np, chk_st, csh)) return;
+  }
   // with prz every wave of the workgroup reaches its barrier: a wave past the
   // last slice reads that slice's header, runs no rounds and writes nothing
   if (s0 >= nslices && !prz) return;
@@ -1343,21 +1387,23 @@ void launch_sell6(const AmgLevel& L, const void* A6, const KT* v, const KT* b, K
 template <typename KT>
 void op_cheb(Ctx* c, int l, const KT* v, const KT* b, KT* r_out, double c1, double c2, KT* d_out,
              const KT* x_base, KT* x_out, const PcgState* st, hipEvent_t ea = nullptr,
-             hipEvent_t eb = nullptr, double* prz = nullptr) {
+             hipEvent_t eb = nullptr, double* prz = nullptr, const double* chk_rr = nullptr) {
   hipStream_t s = c->stream;
   if (l == 0) {
+    // chk_rr: the PCG's stop check runs in this kernel's prologue
+    PcgState* wst = chk_rr ? const_cast<PcgState*>(st) : nullptr;
     const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
     const dim3 grid(nblocks(c->nslices, 4));
     if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
       hipExtLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, ea, eb, 0, c->d_sell_ptr,
                             c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
                             r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                            c->nslices, c->nn, st, prz);
+                            c->nslices, c->nn, st, prz, chk_rr, c->np, wst);
     else
       hipLaunchKernelGGL((k_sell_cheb<KT, KT>), grid, dim3(256), 0, s, c->d_sell_ptr,
                          c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell, v, b,
                          r_out, (const KT*)c->d_Dinv, (KT)c1, (KT)c2, d_out, x_base, x_out,
-                         c->nslices, c->nn, st, prz);
+                         c->nslices, c->nn, st, prz, chk_rr, c->np, wst);
   } else {
     const AmgLevel& L = c->amg.lev[l];
     using CT = typename CoarseSmootherType<KT>::type;
@@ -1487,7 +1533,7 @@ void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, 
 
 template <typename KT>
 int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hipEvent_t* ev,
-             double* prz) {
+             double* prz, const double* chk_rr) {
   auto evs = [&](int l, int j) { return l == 0 && ev ? ev[j] : nullptr; };
   Amg& A = c->amg;
   hipStream_t s = c->stream;
@@ -1508,11 +1554,12 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     if (degree1(l)) {
       // degree 1: x = d0 itself (prolonged into in place), the exact r = b - A d0
       op_cheb<KT>(c, l, (const KT*)L.d_d0, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st,
-                  evs(l, 0), evs(l, 1));
+                  evs(l, 0), evs(l, 1), nullptr, l == 0 ? chk_rr : nullptr);
     } else {
       KT* x = (KT*)L.d_x;
       // x = d0 + d1, then r = b - A x
-      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st);
+      op_cheb<KT>(c, l, (const KT*)L.d_d0, b, nullptr, ch.c1, ch.c2, (KT*)L.d_d1, nullptr, x, st,
+                  nullptr, nullptr, nullptr, l == 0 ? chk_rr : nullptr);
       op_cheb<KT>(c, l, x, b, r, 0.0, 0.0, nullptr, nullptr, nullptr, st, evs(l, 0), evs(l, 1));
     }
     AmgLevel& C = A.lev[l + 1];
@@ -1571,12 +1618,14 @@ int amg_numeric(Ctx* c) {
 }
 
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
-                  hipEvent_t* ev, double* prz) {
+                  hipEvent_t* ev, double* prz, const double* chk_rr) {
   if (prz && c->amg.fine_degree != 1)
     return set_err(c, ONSAS_ERR_STATE, "r.z partials come from the degree-1 fine smoothing only");
+  if (chk_rr && !d0_ready)
+    return set_err(c, ONSAS_ERR_STATE, "the stop check rides on the cycle's first tangent product");
   return c->precision == ONSAS_PRECISION_FP64
-             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready, ev, prz)
-             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev, prz);
+             ? vcycle_t<double>(c, (const double*)b, (double*)x, st, d0_ready, ev, prz, chk_rr)
+             : vcycle_t<float>(c, (const float*)b, (float*)x, st, d0_ready, ev, prz, chk_rr);
 }
 
 double amg_fine_smoother(Ctx* c, void** d0) {

@@ -393,8 +393,12 @@ int amg_numeric(Ctx* c);                                        // Galerkin, inv
 // cycle's two exact fine-level products (r = b - A x), in cycle order
 // prz (optional, degree-1 fine smoothing only): one r.z partial per fine-level
 // slice, b.x of the cycle's input and output, written by its last product
+// chk_rr (optional, with d0_ready): the PCG's r.r partials; the cycle's first
+// tangent product then runs the PCG's stop check in its prologue (and writes
+// the stop into st), so a converged PCG skips the rest of the cycle
 int launch_vcycle(Ctx* c, const void* b, void* x, const PcgState* st, bool d0_ready,
-                  hipEvent_t* ev = nullptr, double* prz = nullptr);
+                  hipEvent_t* ev = nullptr, double* prz = nullptr,
+                  const double* chk_rr = nullptr);
 double amg_fine_smoother(Ctx* c, void** d0);  // that buffer and its scale 1/theta
 void amg_free(Ctx* c);
 int amg_nlevels(Ctx* c);  // hierarchy inspection (0 before it is built)

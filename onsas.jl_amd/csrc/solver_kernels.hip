@@ -854,13 +854,12 @@ __global__ void k_residual_kt(const double* __restrict__ b, const KT* __restrict
   v[k] = ((mask[k / 3] >> (k % 3)) & 1u) ? KT(0) : (KT)(b[k] - (double)Kx[k]);
 }
 
-// One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
-// optional events around pcg_A.
-// Convergence check right after the update (multigrid PCG): ||r_{k+1}||^2 from
-// the update's partials, summed exactly as the next k_pcg_A sums them (same
-// helper, same width), and the same stopping rule, so the iteration count is
-// unchanged; a converged solve then skips the preconditioner application of
-// its last residual instead of computing it and stopping one kernel later.
+// Convergence check of the initial residual (multigrid PCG): ||r_0||^2 from the
+// initialisation's partials, summed exactly as the first k_pcg_A sums them
+// (same helper, same width), and the same stopping rule, so the iteration
+// count is unchanged; a (warm) start that already meets the tolerance skips
+// the preconditioner application.  In the iterations the cycle's first tangent
+// product runs the same check (pcg_stop_check).
 template <int T>
 __global__ __launch_bounds__(T) void k_pcg_check(const double* __restrict__ prr,
                                                  PcgState* __restrict__ st, int np) {
@@ -892,6 +891,8 @@ inline int rz_count(const Ctx* c) {
   return rz_from_cycle(c) ? (int)((c->nslices + 3) / 4) : c->np;  // per 4-slice workgroup
 }
 
+// One PCG iteration of parity `par` (0 for even k): pcg_A then pcg_B, with
+// optional events around pcg_A.
 template <typename KT>
 void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEvent_t* evc = nullptr) {
   hipStream_t s = c->stream;
@@ -918,10 +919,11 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                        (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
                        (const KT*)c->d_Dinv, (KT)s0, d0, c->np, nrz, c->chunk, c->nn, par);
-    hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
-                       part(c, kPartRR), st, c->np);
+    // the stop check (||r_{k+1}||^2 from the update's partials) runs in the
+    // prologue of the cycle's first tangent product
     const bool fused = rz_from_cycle(c);
-    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc, fused ? RZ[1 - par] : nullptr);
+    (void)launch_vcycle(c, c->d_pr, c->d_pz, st, true, evc, fused ? RZ[1 - par] : nullptr,
+                        part(c, kPartRR));
     if (!fused)
       hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(c->np), dim3(NT), 0, s, (const KT*)c->d_pr,
                          (const KT*)c->d_pz, RZ[1 - par], st, c->np, c->chunk, c->nn);
