@@ -267,9 +267,9 @@ int onsas_amg_get_prolongator(onsas_ctx* ctx, int32_t level, int32_t* p_ptr, int
 int onsas_time_cycle_products(onsas_ctx* ctx, int32_t launches, double* ms_per_launch);
 /* The same measurement for any smoothed level (no counterpart in the
  * reference): level 0 as onsas_time_cycle_products; a level >= 1 stored in
- * sliced ELL cycles through its four products of the V-cycle (x = d0 + d1,
- * r = b - A x, r and d0 = D^-1 r / theta, x += d0 + d1) on that level's work
- * vectors.  Levels without a sliced-ELL operator and the coarsest level are
+ * sliced ELL cycles through its products of the V-cycle on that level's work
+ * vectors: the four of the degree-2 smoothing (x = d0 + d1, r = b - A x, r and
+ * d0 = D^-1 r / theta, x += d0 + d1), or the two of the degree-1 step.  Levels without a sliced-ELL operator and the coarsest level are
  * rejected (ONSAS_ERR_ARG). */
 int onsas_time_level_products(onsas_ctx* ctx, int32_t level, int32_t launches,
                               double* ms_per_launch);
