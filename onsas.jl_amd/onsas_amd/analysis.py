@@ -15,6 +15,7 @@ libonsas_hip through MI355XBackend.
 from __future__ import annotations
 
 import copy
+import json
 import time
 import warnings
 from dataclasses import dataclass, field
@@ -224,6 +225,7 @@ class Solution:
         self.strain = [None] * len(sa.load_factors)
         self.records = [None] * len(sa.load_factors)
         self.newton_log = []  # one dict per Newton iteration
+        self.step_wall_s = []  # host wall time of each load step
         self.wall_s = 0.0
 
     def displacements(self, step=None, node=None, component=None):
@@ -243,6 +245,26 @@ class Solution:
 
     def linear_iterations(self):
         return sum(d["lin_iters"] for d in self.newton_log)
+
+    def metrics(self):
+        """One record per stored load step: the reference's per-step table
+        (Solutions.jl:106-134: ||U||, ||dU||/||U||, ||dR||, ||dR||/||Fext||,
+        criterion, Newton iterations) plus the engine's inner solves (iterations,
+        GMRES fallbacks, tolerances) and phase times (device ms, host wall s)."""
+        out = []
+        for k, r in enumerate(self.records):
+            if r is None:
+                continue
+            its = [d for d in self.newton_log if d.get("load_step") == k + 1]
+            out.append(step_metrics(k + 1, self.analysis.load_factors[k], self.U[k], r, its,
+                                    self.step_wall_s[k] if k < len(self.step_wall_s) else None))
+        return out
+
+    def write_jsonl(self, path):
+        """The per-step records as JSON lines."""
+        with open(path, "w") as f:
+            for m in self.metrics():
+                f.write(json.dumps(m) + "\n")
 
     def table(self):
         """Rows of _print_table (Solutions.jl:106-134)."""
@@ -300,6 +322,29 @@ def store_(sol: Solution, sa: NonLinearStaticAnalysis, step: int, ustore: _UStor
                                  dr_rel=ri.dr_rel, iterations=ri.iter, criterion=ri.criterion)
 
 
+_LIN_SOLVER = {1: "pcg", 2: "gmres", 3: "pcg+gmres"}
+
+
+def step_metrics(step, t, U, rec, its, wall_s):
+    """The JSON record of one load step (see Solution.metrics); its: the step's
+    Newton iterations as newton_step returned them."""
+    crit = rec["criterion"]
+    return dict(
+        step=int(step), time=float(t), U_norm=float(np.linalg.norm(U)) if U is not None else None,
+        dU_rel=float(rec["dU_rel"]), dr_norm=float(rec["dr_norm"]), dr_rel=float(rec["dr_rel"]),
+        criterion=type(crit).__name__ if not isinstance(crit, str) else crit,
+        newton_iterations=int(rec["iterations"]),
+        lin_iters=int(sum(d["lin_iters"] for d in its)),
+        lin_iters_per_iteration=[int(d["lin_iters"]) for d in its],
+        lin_solver=[_LIN_SOLVER.get(int(d["lin_solver"]), "none") for d in its],
+        gmres_fallbacks=int(sum(1 for d in its if int(d["lin_solver"]) == 3)),
+        lin_rtol=[float("%.3e" % d["lin_rtol"]) for d in its],
+        lin_converged=all(bool(d["lin_converged"]) for d in its),
+        assemble_ms=round(float(sum(d["assemble_ms"] for d in its)), 4),
+        solve_ms=round(float(sum(d["solve_ms"] for d in its)), 4),
+        wall_s=None if wall_s is None else round(float(wall_s), 6))
+
+
 # ---- the hot path ---------------------------------------------------------------
 
 def assemble_(s, sa):
@@ -333,18 +378,29 @@ def _solve_nonlinear_(sa, alg, backend):
     sol = Solution(sa, alg)
     t0 = time.perf_counter()
     ustore = _UStore(sa.state)
+    jsonl = open(backend.metrics_jsonl, "w") if backend.metrics_jsonl else None
     try:
         while not sa.is_done():
             step = sa.current_step
+            ts = time.perf_counter()
             reset_(sa.iter_state)
             sa.state.apply_loads(sa.current_time())
             while isinstance(isconverged_(sa.iter_state, tolerances(alg)), NotConvergedYet):
                 assemble_(s, sa)
                 step_(sa, alg, backend, sol)
             store_(sol, sa, step, ustore)
+            sol.step_wall_s.append(time.perf_counter() - ts)
+            if jsonl is not None:  # the step's record as it completes (U's norm comes later)
+                its = [d for d in sol.newton_log if d.get("load_step") == step]
+                jsonl.write(json.dumps(step_metrics(step, sa.current_time(), None,
+                                                    sol.records[step - 1], its,
+                                                    sol.step_wall_s[-1])) + "\n")
+                jsonl.flush()
             sa.next_()
     finally:
         ustore.finish()  # every stored U is complete before the solution is returned
+        if jsonl is not None:
+            jsonl.close()
     sol.wall_s = time.perf_counter() - t0
     return sol
 

@@ -33,7 +33,9 @@ class MI355XBackend:
     aggregation multigrid V-cycle with rigid-body modes and Chebyshev smoothing).
     amg_refresh: "step" (coarse levels rebuilt once per load step), "assembly"
     (after every assembly) or "once" (built from the first tangent only); the
-    finest level is always the current tangent."""
+    finest level is always the current tangent.  metrics_jsonl: a file that
+    receives one JSON record per load step while the solve runs (the fields of
+    Solution.metrics, with U_norm null)."""
 
     precision: str = "mixed"
     rtol: float = None
@@ -49,6 +51,9 @@ class MI355XBackend:
     eta_first: float = 5e-3  # the fastest first inner tolerance on the 64^3 cube, both modes
     preconditioner: str = "block-jacobi"
     amg_refresh: str = "step"
+    # path of a JSON-lines file that receives one record per load step as the
+    # solve goes (Solution.metrics gives the same records afterwards)
+    metrics_jsonl: str = None
 
     def precision_code(self):
         if self.precision not in ("mixed", "fp64"):

@@ -279,6 +279,29 @@ def _newton(structure, nsteps, prec, tol_f=1e-10, **kw):
     return solve(sa, alg, be)
 
 
+def test_metrics_jsonl_streamed(tmp_path):
+    """MI355XBackend(metrics_jsonl=...) writes one record per load step while
+    the solve runs; the records equal Solution.metrics() (U's norm aside, which
+    the streamed records leave null) and their inner iterations add up."""
+    import json
+    print("[gpu] metrics jsonl", flush=True)
+    s, sm = workloads.cube_extension(n=4)
+    path = tmp_path / "steps.jsonl"
+    sol = _newton(s, 4, "mixed", preconditioner="amg", forcing=True, metrics_jsonl=str(path))
+    lines = [json.loads(x) for x in path.read_text().splitlines()]
+    m = sol.metrics()
+    assert len(lines) == len(m) == 4
+    for a, b in zip(lines, m):
+        assert a["U_norm"] is None and b["U_norm"] > 0
+        assert {k: v for k, v in a.items() if k != "U_norm"} == \
+            {k: v for k, v in b.items() if k != "U_norm"}
+        assert b["newton_iterations"] == len(b["lin_iters_per_iteration"]) or \
+            b["newton_iterations"] == len(b["lin_iters_per_iteration"]) + 1
+        assert b["dr_rel"] <= 1e-10 and b["wall_s"] > 0
+    assert sum(b["lin_iters"] for b in m) == sol.linear_iterations()
+    assert [b["step"] for b in m] == [1, 2, 3, 4]
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_uniaxial_extension_cube(prec):
     """SVK cube 4^3 x 6 tets, tension 3e5 Pa, 10 steps, tol 1e-10: every load
