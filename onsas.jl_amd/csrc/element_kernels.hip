@@ -662,6 +662,112 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs ga, const SellGroup* 
                 f2);
 }
 
+// pair_index(min(a, b), max(a, b)) as nibble 4 a + b (a lookup with no branches)
+constexpr uint64_t kPairTable = 0x9863875265413210ull;
+constexpr bool pair_table_ok() {
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      if ((int)((kPairTable >> (16 * a + 4 * b)) & 15u) != lo * 4 - lo * (lo - 1) / 2 + (hi - lo))
+        return false;
+    }
+  return true;
+}
+static_assert(pair_table_ok(), "pair table");
+
+// base[idx] through a 32-bit byte offset off a wave-uniform base (the host
+// checks that every plane offset fits), which the compiler turns into a scalar
+// base plus one 32-bit lane register per load
+template <typename T>
+__device__ inline T ld32(const T* __restrict__ base, uint32_t idx) {
+  return *(const T*)((const char*)base + idx * (uint32_t)sizeof(T));
+}
+
+// One contribution of a model with a single tet group (entries carry the
+// element's position in the group: its offset is 0).
+template <typename KT, bool HM>
+__device__ inline void load_tet(const KT* __restrict__ st, const double* __restrict__ fe, uint32_t ne,
+                                uint32_t en, bool dg, Contrib<KT, HM>& c) {
+  const bool valid = en != kNoContrib;
+  const uint32_t ev = valid ? en : 0u;
+  const uint32_t e = ev >> 4;
+  const int a = (ev >> 2) & 3, b = ev & 3;
+  c.w = valid ? KT(1) : KT(0);
+  c.tet = 1;
+  c.hasM = HM;
+  c.c1 = ld32(st, e);
+  c.c2 = ld32(st, ne + e);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    c.va[i] = ld32(st, (uint32_t)(2 + 3 * a + i) * ne + e);
+    c.vb[i] = ld32(st, (uint32_t)(2 + 3 * b + i) * ne + e);
+  }
+  const int p = (int)((kPairTable >> (16 * a + 4 * b)) & 15u);
+  c.g = ld32(st, (uint32_t)(14 + p) * ne + e);
+  c.h = KT(0);
+  if (HM) {
+    c.h = ld32(st, (uint32_t)(24 + p) * ne + e);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) c.M[q] = ld32(st, (uint32_t)(34 + q) * ne + e);
+  }
+  c.f[0] = c.f[1] = c.f[2] = 0.0;
+  if (dg && valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c.f[d] = ld32(fe, (uint32_t)(3 * a + d) * ne + e);
+  }
+}
+
+// k_gather for a model of one tet group: the same contributions added in the
+// same order, the group's planes at wave-uniform bases with 32-bit offsets (no
+// per-lane group lookup or 64-bit address arithmetic), and NC contributions
+// per step, so that each memory round trip of a wave carries NC x 9 state loads.
+template <typename KT, bool HM, int NC>
+__global__ __launch_bounds__(256) void k_gather_tet(
+    const KT* __restrict__ st, const double* __restrict__ fe, uint32_t ne,
+    const SellGroup* __restrict__ groups, int64_t ngroups, const uint32_t* __restrict__ contrib,
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const uint8_t* __restrict__ mask, KT* __restrict__ Ks, KT* __restrict__ Dinv,
+    double* __restrict__ fint, int64_t nn) {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t L = (b & 7) * (nb >> 3) + (b >> 3);  // XCD-aware, as k_gather
+  // wave-uniform (scalar loads of the group record, scalar loop control)
+  const int32_t w = __builtin_amdgcn_readfirstlane((int32_t)(L * 4 + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  if (w >= ngroups) return;
+  const SellGroup sg = groups[w];
+  const int64_t r = sg.row0 + lane % kSellRows;
+  const int k = kSellLanes * sg.round + lane / kSellRows;
+  if (r >= nn) return;
+  const int32_t rs = row_ptr[r];
+  if (k >= row_ptr[r + 1] - rs) return;
+  const int32_t cc = col[rs + k];
+  const bool dg = (cc == r);
+  KT acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = KT(0);
+  double f0 = 0, f1 = 0, f2 = 0;
+  const uint32_t* cp = contrib + sg.cbase + lane;
+  const int qw = sg.qwidth;
+  uint32_t nx[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) nx[j] = j < qw ? cp[64 * j] : kNoContrib;
+  for (int q = 0; q < qw; q += NC) {
+    uint32_t ex[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      ex[j] = nx[j];
+      nx[j] = q + NC + j < qw ? cp[64 * (int64_t)(q + NC + j)] : kNoContrib;
+    }
+    Contrib<KT, HM> cx[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) load_tet(st, fe, ne, ex[j], dg, cx[j]);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) add_contrib(cx[j], acc, f0, f1, f2);
+  }
+  gather_finish(acc, mask[r], mask[cc], dg, Ks + 9 * (int64_t)sg.base + lane, Dinv, fint, r, f0, f1,
+                f2);
+}
+
 // ---- staged gather ---------------------------------------------------------------
 // One workgroup per `rows` consecutive rows of a 64-row slice (the lists of
 // host_setup.hpp::staged_gather).  First the states of the workgroup's elements
@@ -1365,7 +1471,25 @@ int assemble_gather_t(Ctx* c) {
   const unsigned gb = (grid_for(ngrp, 4) + 7) / 8 * 8;  // a multiple of 8 (XCD mapping)
   bool hm = false;
   for (const ElemGroup& g : c->groups) hm = hm || g.hasM;
-  if (hm)
+  const ElemGroup& g0 = c->groups[0];
+  const bool one_tet = ga.ng == 1 && g0.type == kTet &&
+                       (uint64_t)kTetStateM * g0.ne * sizeof(KT) < (uint64_t(1) << 32) &&
+                       (uint64_t)12 * g0.ne * sizeof(double) < (uint64_t(1) << 32);
+  // contributions per step: 4 in FP32 (93 VGPRs, 5 waves per SIMD), 2 in FP64
+  // (on the 64^3 cube: 0.234 / 0.242 / 0.275 ms for 4 / 2 / the general kernel
+  // in FP32, 0.370 / 0.383 / 0.388 ms for 2 / 4 / general in FP64)
+  constexpr int kNC = sizeof(KT) == 4 ? 4 : 2;
+  if (one_tet && hm)
+    hipLaunchKernelGGL((k_gather_tet<KT, true, kNC>), dim3(gb), dim3(256), 0, s,
+                       (const KT*)g0.d_state, (const double*)g0.d_fe, (uint32_t)g0.ne,
+                       c->d_sell_groups, ngrp, c->d_contrib, c->d_row_ptr, c->d_col, c->d_fixmask,
+                       (KT*)c->d_Ksell, (KT*)c->d_Dinv, c->d_fint, c->nn);
+  else if (one_tet)
+    hipLaunchKernelGGL((k_gather_tet<KT, false, kNC>), dim3(gb), dim3(256), 0, s,
+                       (const KT*)g0.d_state, (const double*)g0.d_fe, (uint32_t)g0.ne,
+                       c->d_sell_groups, ngrp, c->d_contrib, c->d_row_ptr, c->d_col, c->d_fixmask,
+                       (KT*)c->d_Ksell, (KT*)c->d_Dinv, c->d_fint, c->nn);
+  else if (hm)
     hipLaunchKernelGGL((k_gather<KT, true>), dim3(gb), dim3(256), 0, s, ga, c->d_sell_groups,
                        ngrp, c->d_contrib, c->d_row_ptr, c->d_col, c->d_fixmask, (KT*)c->d_Ksell,
                        (KT*)c->d_Dinv, c->d_fint, c->nn);
