@@ -366,9 +366,69 @@ __global__ __launch_bounds__(kScatterThreads) void k_truss_scatter(TrussArgs ta,
 //   g_ab = V sigma_ab, h_ab = V c2 s_ab (SVK),
 // rounded once to KT.  Pass 2 (one lane per stored tangent block) sums the
 // blocks listed for it and writes it once.
-__device__ inline int pair_index(int a, int b) {  // a <= b over 4 nodes -> 0..9
+__host__ __device__ constexpr int pair_index(int a, int b) {  // a <= b over 4 nodes -> 0..9
   return a * 4 - a * (a - 1) / 2 + (b - a);
 }
+
+// pair_index(min(a, b), max(a, b)) as nibble 4 a + b (a lookup with no branches)
+constexpr uint64_t kPairTable = 0x9863875265413210ull;
+// the off-diagonal pairs 01 02 03 12 13 23 -> 0..5, nibble 4 a + b (a != b)
+constexpr uint64_t kOffPairTable = 0x0542503143002100ull;
+constexpr bool pair_tables_ok() {
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      if ((int)((kPairTable >> (16 * a + 4 * b)) & 15u) != pair_index(lo, hi)) return false;
+      const int j = lo == 0 ? hi - 1 : (lo == 1 ? hi + 1 : 5);
+      if (a != b && (int)((kOffPairTable >> (16 * a + 4 * b)) & 15u) != j) return false;
+    }
+  return true;
+}
+static_assert(pair_tables_ok(), "pair tables");
+
+// 4 and 2 consecutive values (16- and 8-byte aligned in FP32, 32 and 16 in FP64)
+__device__ inline void st4(float* p, float a, float b, float c, float d) {
+  *(float4*)p = make_float4(a, b, c, d);
+}
+__device__ inline void st4(double* p, double a, double b, double c, double d) {
+  ((double2*)p)[0] = make_double2(a, b);
+  ((double2*)p)[1] = make_double2(c, d);
+}
+__device__ inline void st2(float* p, float a, float b) { *(float2*)p = make_float2(a, b); }
+__device__ inline void st2(double* p, double a, double b) { *(double2*)p = make_double2(a, b); }
+
+// Where the tet state lives (stride ne, element e), by logical plane p as
+// numbered below: node a's record (v_a0, v_a1, v_a2, g_aa) at 4 (a ne + e),
+// (V c1, V c2) at 16 ne + 2 e, the off-diagonal g_ab at (18 + j) ne + e with
+// j = 0..5 for ab = 01 02 03 12 13 23, and SVK's h_ab and M at p ne + e
+// (p >= 24).  The lanes of a wave read consecutive elements, so one 16-byte
+// load (FP32) brings a node's v_a with its g_aa and one 8-byte load c1 and c2.
+__host__ __device__ constexpr int64_t tet_plane(int p, int64_t ne, int64_t e) {
+  if (p < 2) return 16 * ne + 2 * e + p;
+  if (p < 14) return 4 * (((p - 2) / 3) * ne + e) + (p - 2) % 3;
+  if (p < 24) {
+    const int q = p - 14;  // pair index
+    const int a = q < 4 ? 0 : (q < 7 ? 1 : (q < 9 ? 2 : 3));
+    const int b = q - pair_index(a, a) + a;
+    if (a == b) return 4 * (a * ne + e) + 3;
+    return (18 + (int)((kOffPairTable >> (16 * a + 4 * b)) & 15u)) * ne + e;
+  }
+  return p * ne + e;
+}
+// every logical plane of the base record lands on its own slot (a bijection
+// onto [0, 24 ne) for ne = 5)
+constexpr bool tet_plane_ok() {
+  bool hit[120] = {};
+  for (int p = 0; p < 24; ++p)
+    for (int e = 0; e < 5; ++e) {
+      const int64_t o = tet_plane(p, 5, e);
+      if (o < 0 || o >= 120 || hit[o]) return false;
+      hit[o] = true;
+    }
+  return tet_plane(3, 5, 2) == 4 * (0 * 5 + 2) + 1 && tet_plane(14 + 4, 5, 1) == 4 * (5 + 1) + 3 &&
+         tet_plane(14 + 5, 5, 1) == (18 + 3) * 5 + 1 && tet_plane(1, 5, 4) == 16 * 5 + 9;
+}
+static_assert(tet_plane_ok(), "tet state layout");
 
 template <typename KT>
 __global__ __launch_bounds__(256) void k_tet_state(TetArgs ta, const double* __restrict__ U,
@@ -389,12 +449,7 @@ __global__ __launch_bounds__(256) void k_tet_state(TetArgs ta, const double* __r
 #pragma unroll
     for (int d = 0; d < 3; ++d) fe[(a * 3 + d) * ne + e] = fa[d];
   }
-  st[e] = (KT)(V * ev.c1);
-  st[ne + e] = (KT)(V * ev.c2);
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) st[(2 + 3 * a + i) * ne + e] = (KT)ev.v[a][i];
+  KT g[10];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -411,12 +466,23 @@ __global__ __launch_bounds__(256) void k_tet_state(TetArgs ta, const double* __r
         sig = n[a][0] * Sn[0] + n[a][1] * Sn[1] + n[a][2] * Sn[2];
       }
       if (hasM) {
-        st[(14 + p) * ne + e] = (KT)(V * sig);
+        g[p] = (KT)(V * sig);
         st[(24 + p) * ne + e] = (KT)(V * ev.c2 * s);
       } else {
-        st[(14 + p) * ne + e] = (KT)(V * (ev.c2 * s + sig));
+        g[p] = (KT)(V * (ev.c2 * s + sig));
       }
     }
+  // the layout of tet_plane, written with vector stores
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    st4(st + 4 * (a * ne + e), (KT)ev.v[a][0], (KT)ev.v[a][1], (KT)ev.v[a][2], g[pair_index(a, a)]);
+  st2(st + 16 * ne + 2 * e, (KT)(V * ev.c1), (KT)(V * ev.c2));
+  st[18 * ne + e] = g[pair_index(0, 1)];
+  st[19 * ne + e] = g[pair_index(0, 2)];
+  st[20 * ne + e] = g[pair_index(0, 3)];
+  st[21 * ne + e] = g[pair_index(1, 2)];
+  st[22 * ne + e] = g[pair_index(1, 3)];
+  st[23 * ne + e] = g[pair_index(2, 3)];
   if (hasM) {
     st[34 * ne + e] = (KT)ev.M[0];
     st[35 * ne + e] = (KT)ev.M[1];
@@ -501,17 +567,17 @@ __device__ inline void load_contrib(const GatherArgs& ga, uint32_t en, bool dg, 
   c.w = valid ? KT(1) : KT(0);
   c.tet = type == kTet;
   c.hasM = hasM;
-  c.c1 = st[e];
-  c.c2 = st[ne + e];
   c.h = KT(0);
   if (c.tet) {
+    c.c1 = st[tet_plane(0, ne, e)];
+    c.c2 = st[tet_plane(1, ne, e)];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      c.va[i] = st[(2 + 3 * a + i) * ne + e];
-      c.vb[i] = st[(2 + 3 * b + i) * ne + e];
+      c.va[i] = st[tet_plane(2 + 3 * a + i, ne, e)];
+      c.vb[i] = st[tet_plane(2 + 3 * b + i, ne, e)];
     }
     const int p = a <= b ? pair_index(a, b) : pair_index(b, a);
-    c.g = st[(14 + p) * ne + e];
+    c.g = st[tet_plane(14 + p, ne, e)];
     if (HM && hasM) {
       c.h = st[(24 + p) * ne + e];
 #pragma unroll
@@ -522,6 +588,8 @@ __device__ inline void load_contrib(const GatherArgs& ga, uint32_t en, bool dg, 
       for (int d = 0; d < 3; ++d) c.f[d] = fp[(3 * a + d) * ne + e];
     }
   } else {
+    c.c1 = st[e];
+    c.c2 = st[ne + e];
     // truss: K_ab = s_a s_b [cI I + cdd d d'] in the same factored form
     const KT sab = (a == b) ? KT(1) : KT(-1);
 #pragma unroll
@@ -662,25 +730,39 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs ga, const SellGroup* 
                 f2);
 }
 
-// pair_index(min(a, b), max(a, b)) as nibble 4 a + b (a lookup with no branches)
-constexpr uint64_t kPairTable = 0x9863875265413210ull;
-constexpr bool pair_table_ok() {
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 4; ++b) {
-      const int lo = a < b ? a : b, hi = a < b ? b : a;
-      if ((int)((kPairTable >> (16 * a + 4 * b)) & 15u) != lo * 4 - lo * (lo - 1) / 2 + (hi - lo))
-        return false;
-    }
-  return true;
-}
-static_assert(pair_table_ok(), "pair table");
-
 // base[idx] through a 32-bit byte offset off a wave-uniform base (the host
 // checks that every plane offset fits), which the compiler turns into a scalar
 // base plus one 32-bit lane register per load
 template <typename T>
 __device__ inline T ld32(const T* __restrict__ base, uint32_t idx) {
   return *(const T*)((const char*)base + idx * (uint32_t)sizeof(T));
+}
+
+// 4 and 2 consecutive values at base[idx] (32-bit byte offsets, as ld32)
+__device__ inline void ld4(const float* __restrict__ base, uint32_t idx, float v[4]) {
+  const float4 t = *(const float4*)((const char*)base + idx * 4u);
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ inline void ld4(const double* __restrict__ base, uint32_t idx, double v[4]) {
+  const double2* q = (const double2*)((const char*)base + idx * 8u);
+  const double2 t0 = q[0], t1 = q[1];
+  v[0] = t0.x;
+  v[1] = t0.y;
+  v[2] = t1.x;
+  v[3] = t1.y;
+}
+__device__ inline void ld2(const float* __restrict__ base, uint32_t idx, float v[2]) {
+  const float2 t = *(const float2*)((const char*)base + idx * 4u);
+  v[0] = t.x;
+  v[1] = t.y;
+}
+__device__ inline void ld2(const double* __restrict__ base, uint32_t idx, double v[2]) {
+  const double2 t = *(const double2*)((const char*)base + idx * 8u);
+  v[0] = t.x;
+  v[1] = t.y;
 }
 
 // One contribution of a model with a single tet group (entries carry the
@@ -695,15 +777,24 @@ __device__ inline void load_tet(const KT* __restrict__ st, const double* __restr
   c.w = valid ? KT(1) : KT(0);
   c.tet = 1;
   c.hasM = HM;
-  c.c1 = ld32(st, e);
-  c.c2 = ld32(st, ne + e);
+  // tet_plane's records: node a's (v_a, g_aa) and (c1, c2) in one vector load
+  // each; node b's record and the off-diagonal g_ab only when b != a
+  KT A[4], C2[2];
+  ld4(st, 4u * ((uint32_t)a * ne + e), A);
+  ld2(st, 16u * ne + 2u * e, C2);
+  c.c1 = C2[0];
+  c.c2 = C2[1];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    c.va[i] = ld32(st, (uint32_t)(2 + 3 * a + i) * ne + e);
-    c.vb[i] = ld32(st, (uint32_t)(2 + 3 * b + i) * ne + e);
+  for (int i = 0; i < 3; ++i) c.va[i] = c.vb[i] = A[i];
+  c.g = A[3];
+  if (a != b) {
+    KT B[4];
+    ld4(st, 4u * ((uint32_t)b * ne + e), B);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c.vb[i] = B[i];
+    c.g = ld32(st, (18u + (uint32_t)((kOffPairTable >> (16 * a + 4 * b)) & 15u)) * ne + e);
   }
   const int p = (int)((kPairTable >> (16 * a + 4 * b)) & 15u);
-  c.g = ld32(st, (uint32_t)(14 + p) * ne + e);
   c.h = KT(0);
   if (HM) {
     c.h = ld32(st, (uint32_t)(24 + p) * ne + e);
@@ -718,9 +809,10 @@ __device__ inline void load_tet(const KT* __restrict__ st, const double* __restr
 }
 
 // k_gather for a model of one tet group: the same contributions added in the
-// same order, the group's planes at wave-uniform bases with 32-bit offsets (no
-// per-lane group lookup or 64-bit address arithmetic), and NC contributions
-// per step, so that each memory round trip of a wave carries NC x 9 state loads.
+// same order, the group's state at a wave-uniform base with 32-bit offsets (no
+// per-lane group lookup or 64-bit address arithmetic), read as tet_plane's
+// vector records (3 to 4 loads per contribution instead of 9 scalar ones), NC
+// contributions per step.
 template <typename KT, bool HM, int NC>
 __global__ __launch_bounds__(256) void k_gather_tet(
     const KT* __restrict__ st, const double* __restrict__ fe, uint32_t ne,
@@ -885,7 +977,7 @@ __global__ __launch_bounds__(NT) void k_gather_lds(
   for (int k = 0; k < KS; ++k) {
     if (ei[k] >= 0) {
 #pragma unroll
-      for (int p = 0; p < NST; ++p) v[k][p] = st[p * ne + ei[k]];
+      for (int p = 0; p < NST; ++p) v[k][p] = st[tet_plane(p, ne, ei[k])];
     }
     if (pi[k] != 0xffffffffu) {
       const int64_t e = pi[k] & 0x0fffffffu;
@@ -1475,10 +1567,10 @@ int assemble_gather_t(Ctx* c) {
   const bool one_tet = ga.ng == 1 && g0.type == kTet &&
                        (uint64_t)kTetStateM * g0.ne * sizeof(KT) < (uint64_t(1) << 32) &&
                        (uint64_t)12 * g0.ne * sizeof(double) < (uint64_t(1) << 32);
-  // contributions per step: 4 in FP32 (93 VGPRs, 5 waves per SIMD), 2 in FP64
-  // (on the 64^3 cube: 0.234 / 0.242 / 0.275 ms for 4 / 2 / the general kernel
-  // in FP32, 0.370 / 0.383 / 0.388 ms for 2 / 4 / general in FP64)
-  constexpr int kNC = sizeof(KT) == 4 ? 4 : 2;
+  // two contributions per step (on the 64^3 cube, gather pass in FP32 / FP64:
+  // 0.217 / 0.337 ms with 2, 0.227 / 0.378 ms with 4, 0.240 ms with 6 and
+  // 0.347 ms with 1; the general kernel 0.275 / 0.388 ms)
+  constexpr int kNC = 2;
   if (one_tet && hm)
     hipLaunchKernelGGL((k_gather_tet<KT, true, kNC>), dim3(gb), dim3(256), 0, s,
                        (const KT*)g0.d_state, (const double*)g0.d_fe, (uint32_t)g0.ne,

@@ -89,10 +89,12 @@ struct ElemGroup {
   LocateGrid locate;     // built on the first point location
 };
 
-// Tet state planes (stride ne): 0 V c1, 1 V c2, 2..13 v[a][i], 14..23 g_ab (the
+// Tet state, logical planes: 0 V c1, 1 V c2, 2..13 v[a][i], 14..23 g_ab (the
 // coefficient of I), SVK only: 24..33 h_ab (the coefficient of M), 34..39 M
-// (00 01 02 11 12 22).  Truss: 0 cdd, 1 cI, 2..4 d.  Pairs a <= b numbered
-// 00 01 02 03 11 12 13 22 23 33.
+// (00 01 02 11 12 22); where each lives in memory is tet_plane
+// (element_kernels.hip: per-node 4-vectors (v_a, g_aa), the (c1, c2) pairs,
+// then planes of stride ne).  Truss planes (stride ne): 0 cdd, 1 cI, 2..4 d.
+// Pairs a <= b numbered 00 01 02 03 11 12 13 22 23 33.
 constexpr int kTetStateBase = 24, kTetStateM = 40, kTrussState = 5;
 constexpr int kMaxGatherGroups = 8;
 // staged gather: threads per workgroup, and the most list entries a thread
