@@ -187,7 +187,8 @@ int onsas_set_linear_solver(onsas_ctx* ctx, const onsas_linear_settings* s);
 /* External forces at the current load factor: apply!(sa, load_bcs)
  * (src/StructuralAnalyses/StructuralAnalyses.jl:228-241).  nodal: n values
  * (GlobalLoad contributions already summed per dof, NULL = 0); pressures: one
- * value p(t) per pressure group (NULL if none). */
+ * value p(t) per pressure group (NULL if none).  Both arrays are read before
+ * the call returns; with pressures only, the device work stays queued. */
 int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure,
                     const double* pressures);
 

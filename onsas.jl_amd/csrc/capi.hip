@@ -270,8 +270,10 @@ int onsas_create(int32_t precision, onsas_ctx** out) {
     return ONSAS_ERR_HIP;
   }
   e = hipStreamCreate(&c->stream);  // default priority
+  if (e == hipSuccess) e = hipStreamCreate(&c->copy_stream);  // default priority
   if (e != hipSuccess) {
     g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete ctx;
     return ONSAS_ERR_HIP;
   }
@@ -288,6 +290,7 @@ void onsas_destroy(onsas_ctx* ctx) {
   if (!ctx) return;
   Ctx* c = &ctx->impl;
   (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   for (auto& g : c->pcg_graph)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   onsas::amg_free(c);
@@ -303,10 +306,14 @@ void onsas_destroy(onsas_ctx* ctx) {
                   c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds,
                   c->d_contrib, c->d_sell_groups, (void*)c->d_stage_groups,
                   (void*)c->d_stage_steps, c->d_stage, c->d_stage_words, c->d_stage_pairs,
-                  c->d_pair_row, c->d_colmask};
+                  c->d_pair_row, c->d_colmask, c->d_snap[0], c->d_snap[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);
+  if (c->h_press) (void)hipHostFree(c->h_press);
+  if (c->ev_press) (void)hipEventDestroy(c->ev_press);
+  for (auto& e : c->ev_snap_dev)
+    if (e) (void)hipEventDestroy(e);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   for (double* h : c->h_snap)
     if (h) (void)hipHostFree(h);
@@ -320,6 +327,7 @@ void onsas_destroy(onsas_ctx* ctx) {
   for (auto& e : c->ev_spmv)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   delete ctx;
 }
 
@@ -461,13 +469,21 @@ int onsas_set_loads(onsas_ctx* ctx, const double* nodal, int32_t npressure, cons
   }
   else
     ONSAS_HIP_TRY(c, hipMemsetAsync(c->d_nodal, 0, n * sizeof(double), c->stream));
-  if (npressure > 0)
-    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_pressures, pressures, npressure * sizeof(double),
+  if (npressure > 0) {
+    // pinned staging, so the upload is asynchronous; the last one has read it
+    if (c->press_pending) ONSAS_HIP_TRY(c, hipEventSynchronize(c->ev_press));
+    std::memcpy(c->h_press, pressures, npressure * sizeof(double));
+    ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_pressures, c->h_press, npressure * sizeof(double),
                                     hipMemcpyHostToDevice, c->stream));
+    ONSAS_HIP_TRY(c, hipEventRecord(c->ev_press, c->stream));
+    c->press_pending = true;
+  }
   ONSAS_TRY(onsas::launch_fext(c));
   ONSAS_TRY(onsas::reset_forcing(c));  // a new load step: no previous inner residual
   c->amg.step_stale = true;            // ... and, by default, new multigrid coarse levels
-  ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  // the nodal values' upload reads h_stage, which the state copies reuse;
+  // without them the step's work stays queued
+  if (nodal) ONSAS_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return ONSAS_OK;
 }
 
@@ -562,9 +578,17 @@ int onsas_snapshot_U(onsas_ctx* ctx, int32_t slot) {
   Ctx* c = C(ctx);
   ONSAS_TRY(check_ready(c));
   if (slot < 0 || slot > 1) return set_err(c, ONSAS_ERR_ARG, "snapshot slot must be 0 or 1");
-  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->h_snap[slot], c->d_U, 3 * c->nn * sizeof(double),
-                                  hipMemcpyDeviceToHost, c->stream));
-  ONSAS_HIP_TRY(c, hipEventRecord(c->ev_snap[slot], c->stream));
+  const size_t bytes = 3 * c->nn * sizeof(double);
+  // the slot's last transfer has left d_snap[slot] before it is overwritten
+  if (c->snap_pending[slot]) ONSAS_HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_snap[slot], 0));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->d_snap[slot], c->d_U, bytes, hipMemcpyDeviceToDevice, c->stream));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev_snap_dev[slot], c->stream));
+  // the transfer to the host runs on the copy stream, beside the next step
+  ONSAS_HIP_TRY(c, hipStreamWaitEvent(c->copy_stream, c->ev_snap_dev[slot], 0));
+  ONSAS_HIP_TRY(c, hipMemcpyAsync(c->h_snap[slot], c->d_snap[slot], bytes, hipMemcpyDeviceToHost,
+                                  c->copy_stream));
+  ONSAS_HIP_TRY(c, hipEventRecord(c->ev_snap[slot], c->copy_stream));
+  c->snap_pending[slot] = true;
   return ONSAS_OK;
 }
 

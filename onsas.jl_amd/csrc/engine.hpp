@@ -301,6 +301,16 @@ struct Ctx {
   double* h_stage = nullptr;  // pinned, 3 nn values: staging of state vector copies
   double* h_snap[2] = {};     // pinned, 3 nn values each: U snapshots (onsas_snapshot_U)
   hipEvent_t ev_snap[2] = {};  // ... their copies' completion
+  // A snapshot copies U on the stream into d_snap (device to device, a few
+  // microseconds), then d_snap to h_snap on copy_stream, so the link transfer
+  // overlaps the next load step instead of holding the stream.
+  double* d_snap[2] = {};
+  hipEvent_t ev_snap_dev[2] = {};       // the device copy is done
+  bool snap_pending[2] = {false, false};  // a transfer from d_snap[slot] was queued
+  hipStream_t copy_stream = nullptr;    // default priority
+  double* h_press = nullptr;            // pinned staging of the pressure values
+  hipEvent_t ev_press = nullptr;        // ... their last upload has read it
+  bool press_pending = false;
   // pressure faces
   int32_t *d_face_conn = nullptr, *d_face_grp = nullptr;  // all faces, 3*nf and nf
   int32_t *d_nf_ptr = nullptr, *d_nf_idx = nullptr;       // node -> faces CSR

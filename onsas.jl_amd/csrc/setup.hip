@@ -417,6 +417,10 @@ int finalize_model(Ctx* c) {
   ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_stage, sizeof(double) * n, hipHostMallocDefault));
   for (double*& h : c->h_snap)
     ONSAS_HIP_TRY(c, hipHostMalloc((void**)&h, sizeof(double) * n, hipHostMallocDefault));
+  for (double*& d : c->d_snap) ONSAS_TRY(dev_alloc(c, (void**)&d, sizeof(double) * n));
+  ONSAS_HIP_TRY(c, hipHostMalloc((void**)&c->h_press,
+                                 sizeof(double) * std::max<size_t>(c->pressure_faces.size(), 1),
+                                 hipHostMallocDefault));
   {
     const char* w = std::getenv("ONSAS_WARM_START");
     c->warm_start = !(w && w[0] == '0');
@@ -483,6 +487,8 @@ int finalize_model(Ctx* c) {
   if (c->gather) ONSAS_TRY(launch_gather_init(c));
   for (auto& e : c->ev) ONSAS_HIP_TRY(c, hipEventCreate(&e));
   for (auto& e : c->ev_snap) ONSAS_HIP_TRY(c, hipEventCreate(&e));
+  for (auto& e : c->ev_snap_dev) ONSAS_HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  ONSAS_HIP_TRY(c, hipEventCreateWithFlags(&c->ev_press, hipEventDisableTiming));
   // timing-only events (read after a stream synchronize): no system-scope fence,
   // so the stamped dispatch pays no cache writeback and invalidation
   for (auto& e : c->ev_spmv)

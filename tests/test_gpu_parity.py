@@ -245,6 +245,11 @@ def test_pressure_loads_match_oracle(oracle):
         for a in range(3):
             ref[3 * tri[a]:3 * tri[a] + 3] += fe[a]
     np.testing.assert_allclose(f, ref, rtol=1e-14, atol=1e-15)
+    # pressure-only loads stay queued (staged through pinned memory): two calls
+    # back to back, the second one's values win
+    c.set_loads(None, [7.0])
+    c.set_loads(None, [2.5])
+    np.testing.assert_allclose(c.get_fext(), ref - nodal, rtol=1e-13, atol=1e-14)
     c.close()
 
 
@@ -873,6 +878,15 @@ def test_u_snapshots():
         got1 = pool.submit(c.read_snapshot, 1, np.empty(3 * sm.nn)).result()
     assert np.array_equal(got0, U0) and np.array_equal(got1, U1)
     assert np.array_equal(c.get_U(), U1)
+    # a slot taken again after it was read holds the U of the new moment, not
+    # the U written after it
+    c.set_U(U0 + 1.0)
+    c.snapshot_U(0)
+    c.set_U(U1 - 1.0)
+    c.assemble()
+    got0 = np.empty(3 * sm.nn)
+    c.read_snapshot(0, got0)
+    assert np.array_equal(got0, U0 + 1.0)
     with pytest.raises(_capi.OnsasError):
         c.snapshot_U(2)
     c.close()
