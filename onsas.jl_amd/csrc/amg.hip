@@ -55,6 +55,13 @@ constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 4 wa
 constexpr size_t kX6Lds = 48 * 1024;
 
 inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+// the same rounded up to a multiple of 8 (grids that xcd_block remaps)
+inline unsigned nblocks8(int64_t n, int t) { return (nblocks(n, t) + 7u) / 8u * 8u; }
+
+// Workgroup b of a grid of nb (a multiple of 8) as the logical index that
+// gives each of the 8 XCDs (hardware workgroup ids go round-robin over them)
+// a contiguous run, so that neighbouring rows' vector gathers share one L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
 
 __device__ inline bool pcg_stopped(const PcgState* st) { return st && (st->done | st->breakdown); }
 
@@ -880,7 +887,9 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
   using PV = PT_t<KT>;
   constexpr int kU = sizeof(KT) == 4 ? 2 : 1;
   __shared__ KT sh[4][6];
-  const int64_t I = ((int64_t)blockIdx.x * 256 + threadIdx.x) / LPA;
+  // LPA < 256: an XCD-aware run of aggregates (the grid is a multiple of 8)
+  const int64_t bl = LPA < 256 ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t I = (bl * 256 + threadIdx.x) / LPA;
   const int l = threadIdx.x % LPA;
   if (LPA < 256 && I >= nc) return;  // LPA 256: one aggregate per workgroup, I < nc
   // the coarse row's D^-1 first (its latency hides under the gathers); an
@@ -1010,7 +1019,8 @@ __global__ __launch_bounds__(256) void k_sa_prolong_node(
     const PT_t<KT>* __restrict__ P, const KT* __restrict__ xc, KT* __restrict__ xf, int64_t nf,
     const PcgState* st) {
   using PV = PT_t<KT>;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // an XCD-aware run of nodes (the grid is a multiple of 8)
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= nf) return;
   KT x0[BS];
 #pragma unroll
@@ -1526,7 +1536,7 @@ void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, 
                        L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
   else
-    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 32>), dim3(nblocks(C.n, 8)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 32>), dim3(nblocks8(C.n, 8)), dim3(256), 0, c->stream,
                        L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
 }
@@ -1592,7 +1602,7 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     KT* z = l == 0 ? z0 : (KT*)L.d_x;
     if (l == 0)
       hipLaunchKernelGGL((k_sa_prolong_node<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
-                         dim3(nblocks(L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
+                         dim3(nblocks8(L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
                          (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
