@@ -10,7 +10,8 @@
 //     orthonormalised per aggregate by modified Gram-Schmidt (twice): the
 //     aggregate's rows of Q form the tentative prolongator T (bs x 6 per node),
 //     R (6 x 6) is the next level's near-nullspace;
-//   * smoothed prolongator P = (I - w D^-1 A) T, w = 4 / (3 lambda_max(D^-1 A))
+//   * smoothed prolongator P = (I - w D^-1 A) T, w = 1.15 x 4 / (3 lambda) with
+//     lambda the power-iteration estimate of lambda_max(D^-1 A)
 //     (block-CSR, bs x 6 blocks: row i has a block for every aggregate holding a
 //     neighbour of i); next level A_c = P^T (A P);
 //   * coarse levels have 6 dofs per node; coarsening goes on until at most 300
@@ -47,6 +48,13 @@ constexpr int kMaxCoarseNodes = 300;  // coarsest level: <= 1800 dofs, inverted 
 constexpr double kChebRatio = 30.0;  // smoothing interval [hi / 30, hi], hi = 1.1 lambda
 constexpr int kPowerFirst = 15;      // power-iteration steps, first estimate
 constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
+// Prolongator smoothing weight w = kProlongWeight * 4 / (3 lambda).  On the
+// 64^3 cube (mixed / FP64 PCG iterations of the Newton solve) the factor 0.85,
+// 1.0, 1.1, 1.15, 1.2, 1.3, 1.45 gave 417/382, 379/350, 368/321, 366/321,
+// 367/321, 373/330, 392/355: past the classical 4 / (3 lambda) the smoothed
+// basis functions spread the rigid-body modes further, up to where the
+// smoothing starts to amplify the top of the spectrum (w = 2 / lambda at 1.5).
+constexpr double kProlongWeight = 1.15;
 constexpr int64_t kSell6MinRows = 2048;  // smaller levels: one wave per block row
 constexpr int kSell6Window = 256;     // rows sorted by length within windows of this size
 constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 4 waves restrict
@@ -1469,7 +1477,7 @@ template <typename KT, int BS>
 void launch_products(Ctx* c, AmgLevel& F, AmgLevel& C, const KT* Af, const KT* Df,
                      const int32_t* colf) {
   hipStream_t s = c->stream;
-  const double w = 4.0 / (3.0 * F.lam);
+  const double w = kProlongWeight * 4.0 / (3.0 * F.lam);
   hipLaunchKernelGGL((k_sa_prolongator<KT, BS>), dim3(nblocks(6 * F.nnz_p, 256)), dim3(256), 0, s,
                      F.d_p_row, F.d_p_col, F.d_agg, F.d_pk_ptr, F.d_pk, colf, Af, Df,
                      (const KT*)F.d_T, w, (PT_t<KT>*)F.d_P, F.d_rt_pos, (PT_t<KT>*)F.d_PT,
