@@ -505,7 +505,9 @@ __global__ __launch_bounds__(NT) void k_pcg_init_r(const double* __restrict__ b,
                                                      const uint8_t* __restrict__ mask,
                                                      KT* __restrict__ x, KT* __restrict__ r,
                                                      double* __restrict__ prr,
-                                                     double* __restrict__ pbb, int np, int64_t chunk,
+                                                     double* __restrict__ pbb,
+                                                     const KT* __restrict__ Dinv, KT s,
+                                                     KT* __restrict__ d0, int np, int64_t chunk,
                                                      int64_t nn) {
   __shared__ double sh[NT / 64];
   int64_t r0, r1;
@@ -513,12 +515,19 @@ __global__ __launch_bounds__(NT) void k_pcg_init_r(const double* __restrict__ b,
   double rr = 0;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
     const uint32_t m = mask[i];
+    KT ri[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const KT ri = ((m >> d) & 1u) ? KT(0) : (KT)b[3 * i + d];
+      ri[d] = ((m >> d) & 1u) ? KT(0) : (KT)b[3 * i + d];
       x[3 * i + d] = KT(0);
-      r[3 * i + d] = ri;
-      rr += (double)ri * (double)ri;
+      r[3 * i + d] = ri[d];
+      rr += (double)ri[d] * (double)ri[d];
+    }
+    if (d0) {  // the multigrid cycle's first smoothing step, d0 = s D^-1 r
+      KT zi[3];
+      jacobi_apply(Dinv + 9 * i, ri, zi);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) d0[3 * i + d] = s * zi[d];
     }
   }
   rr = block_sum(rr, sh);
@@ -564,12 +573,13 @@ __global__ __launch_bounds__(NT) void k_warm_dots(const double* __restrict__ b,
 // the scale that minimises ||b - a K x0||: ||r_0|| <= ||b||, so a warm start
 // is never worse than the zero start, and a = 0 reproduces it exactly.
 // Partials r.r and b.b (the stopping rule stays relative to ||b||); with Dinv
-// (block-Jacobi PCG) also z = D^-1 r and r.z.
+// also z = zs D^-1 r and r.z (block-Jacobi PCG: zs = 1; the multigrid cycle's
+// first smoothing step: zs = 1 / theta, z its d0).
 template <typename KT>
 __global__ __launch_bounds__(NT) void k_warm_init(
     const double* __restrict__ b, const uint8_t* __restrict__ mask, const KT* __restrict__ x0,
     const KT* __restrict__ t, const double* __restrict__ ptb, const double* __restrict__ ptt,
-    const KT* __restrict__ Dinv, KT* __restrict__ x, KT* __restrict__ r, KT* __restrict__ z,
+    const KT* __restrict__ Dinv, KT zs, KT* __restrict__ x, KT* __restrict__ r, KT* __restrict__ z,
     double* __restrict__ prz, double* __restrict__ prr, double* __restrict__ pbb, int np,
     int64_t chunk, int64_t nn) {
   __shared__ double sh[3 * NT / 64];
@@ -597,6 +607,7 @@ __global__ __launch_bounds__(NT) void k_warm_init(
       jacobi_apply(Dinv + 9 * i, ri, zi);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
+        zi[d] *= zs;
         z[3 * i + d] = zi[d];
         rz += (double)ri[d] * (double)zi[d];
       }
@@ -862,29 +873,6 @@ __global__ void k_residual_kt(const double* __restrict__ b, const KT* __restrict
   v[k] = ((mask[k / 3] >> (k % 3)) & 1u) ? KT(0) : (KT)(b[k] - (double)Kx[k]);
 }
 
-// Convergence check of the initial residual (multigrid PCG): ||r_0||^2 from the
-// initialisation's partials, summed exactly as the first k_pcg_A sums them
-// (same helper, same width), and the same stopping rule, so the iteration
-// count is unchanged; a (warm) start that already meets the tolerance skips
-// the preconditioner application.  In the iterations the cycle's first tangent
-// product runs the same check (pcg_stop_check).
-template <int T>
-__global__ __launch_bounds__(T) void k_pcg_check(const double* __restrict__ prr,
-                                                 PcgState* __restrict__ st, int np) {
-  __shared__ double sh[3 * T / 64];
-  if (st->done | st->breakdown) return;
-  double unused0, rr, unused2;
-  sum_parts3<T>(prr, prr, nullptr, np, np, sh, unused0, rr, unused2);
-  const int k = st->k;  // iterations completed
-  const bool conv = rr <= st->tol2;
-  if (threadIdx.x == 0 && (conv || k >= st->maxiter)) {
-    st->done = 1;
-    st->iters = k;
-    st->converged = conv ? 1 : 0;
-    st->rr_last = rr;
-  }
-}
-
 // r.z partials of parity par: with the multigrid's degree-1 fine smoothing one
 // per workgroup (4 slices) of the cycle's last fine-level product, which
 // writes them; otherwise one per reduction workgroup
@@ -990,6 +978,10 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
       c->amg.step_stale = false;
     }
   }
+  // multigrid: the init kernels also take the cycle's first smoothing step
+  // d0 = D^-1 r_0 / theta into the finest level's d0
+  void* amg_d0 = nullptr;
+  const KT d0_scale = amg ? (KT)amg_fine_smoother(c, &amg_d0) : KT(1);
   if (x0) {
     // warm start: t = K x0 (into q, free until the first iteration), the scale
     // a minimising ||b - a t||, then x = a x0 and r = b - a t
@@ -1001,13 +993,13 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
                        (const KT*)c->d_pq, part(c, kPartPQ), part(c, kPartRZ1), np, c->chunk, c->nn);
     hipLaunchKernelGGL(k_warm_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
                        (const KT*)x0, (const KT*)c->d_pq, part(c, kPartPQ), part(c, kPartRZ1),
-                       amg ? nullptr : (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr,
-                       (KT*)c->d_pz, part(c, kPartRZ0), part(c, kPartRR), part(c, kPartBB), np,
-                       c->chunk, c->nn);
+                       (const KT*)c->d_Dinv, d0_scale, (KT*)c->d_px, (KT*)c->d_pr,
+                       amg ? (KT*)amg_d0 : (KT*)c->d_pz, part(c, kPartRZ0), part(c, kPartRR),
+                       part(c, kPartBB), np, c->chunk, c->nn);
   } else if (amg) {
     hipLaunchKernelGGL(k_pcg_init_r<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
-                       (KT*)c->d_px, (KT*)c->d_pr, part(c, kPartRR), part(c, kPartBB), np, c->chunk,
-                       c->nn);
+                       (KT*)c->d_px, (KT*)c->d_pr, part(c, kPartRR), part(c, kPartBB),
+                       (const KT*)c->d_Dinv, d0_scale, (KT*)amg_d0, np, c->chunk, c->nn);
   } else {
     hipLaunchKernelGGL(k_pcg_init<KT>, dim3(np), dim3(NT), 0, s, b_dev, c->d_fixmask,
                        (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz,
@@ -1019,13 +1011,12 @@ int pcg_solve_t(Ctx* c, const double* b_dev, bool forcing, const void* x0, int e
                      c->lin.newton_tol, c->lin.eta_max > 0 ? c->lin.eta_max : 0.1,
                      c->lin.eta_first > 0 ? c->lin.eta_first : 1e-3, kForcingGuard);
   if (amg) {
-    // a (warm) start that already meets the tolerance stops here, before the
-    // preconditioner application of r_0 (same rule as the first k_pcg_A)
-    hipLaunchKernelGGL(k_pcg_check<kSpmvThreads>, dim3(1), dim3(kSpmvThreads), 0, s,
-                       part(c, kPartRR), &c->d_sc->pcg, np);
+    // a (warm) start that already meets the tolerance stops in the prologue of
+    // the cycle's first fine-level product, before the preconditioner
+    // application of r_0 (the rule of every later check and of k_pcg_A)
     const bool fused = rz_from_cycle(c);
-    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, false, nullptr,
-                            fused ? rz_parts(c, 0) : nullptr));
+    ONSAS_TRY(launch_vcycle(c, c->d_pr, c->d_pz, &c->d_sc->pcg, true, nullptr,
+                            fused ? rz_parts(c, 0) : nullptr, part(c, kPartRR)));
     if (!fused)
       hipLaunchKernelGGL(k_pcg_rz<KT>, dim3(np), dim3(NT), 0, s, (const KT*)c->d_pr,
                          (const KT*)c->d_pz, part(c, kPartRZ0), &c->d_sc->pcg, np, c->chunk,
