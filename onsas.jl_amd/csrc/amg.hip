@@ -1017,18 +1017,25 @@ __global__ void k_sa_prolong(const int32_t* __restrict__ p_ptr, const int32_t* _
   xf[t] = x0 + acc;
 }
 
-// The same product with one thread per node (all BS dofs): each coarse block
-// of x is gathered once for the node's BS rows, and a third as many waves (the
-// fine level, BS = 3) run in one generation.  kPass blocks per pass as above;
-// every dof's sum keeps the block order of the one-thread-per-dof kernel.
+// The same product with two threads per node (all BS dofs): each coarse
+// block of x is gathered once for the node's BS rows.  The pair's even thread
+// takes the row's blocks 0, 2, 4, ..., the odd one 1, 3, 5, ..., kPass a pass
+// each with clamped, unconditional index loads (a slot past the row's end adds
+// an exact zero), and the even thread adds the two sums in that order and
+// writes the node's values.  One pass covers 2 kPass blocks (a row has about
+// seven), so a row's chain of dependent loads is one pass long (on the 64^3
+// cube 12.5 us against 16.0 us with one thread and two passes per node).
 template <typename KT, int BS, int kPass>
-__global__ __launch_bounds__(256) void k_sa_prolong_node(
+__global__ __launch_bounds__(256) void k_sa_prolong_pair(
     const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
     const PT_t<KT>* __restrict__ P, const KT* __restrict__ xc, KT* __restrict__ xf, int64_t nf,
     const PcgState* st) {
   using PV = PT_t<KT>;
   // an XCD-aware run of nodes (the grid is a multiple of 8)
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 1;
+  const int h = (int)(t & 1);
+  // both threads of a pair are in one wave and take the same exits
   if (i >= nf) return;
   KT x0[BS];
 #pragma unroll
@@ -1038,15 +1045,15 @@ __global__ __launch_bounds__(256) void k_sa_prolong_node(
   KT acc[BS];
 #pragma unroll
   for (int a = 0; a < BS; ++a) acc[a] = KT(0);
-  for (int32_t p = p0; p < p1; p += kPass) {
+  for (int32_t p = p0 + h; p < p1; p += 2 * kPass) {
     int32_t cj[kPass];
 #pragma unroll
-    for (int k = 0; k < kPass; ++k) cj[k] = p_col[min(p + k, p1 - 1)];
+    for (int k = 0; k < kPass; ++k) cj[k] = p_col[min(p + 2 * k, p1 - 1)];
     PV pv[kPass][BS * 6];
     KT xv[kPass][6];
 #pragma unroll
     for (int k = 0; k < kPass; ++k) {  // unconditional loads (clamped addresses)
-      const PV* pp = (const PV*)__builtin_assume_aligned(P + (int64_t)BS * 6 * min(p + k, p1 - 1),
+      const PV* pp = (const PV*)__builtin_assume_aligned(P + (int64_t)BS * 6 * min(p + 2 * k, p1 - 1),
                                                          2 * sizeof(PV));
       const KT* xj = (const KT*)__builtin_assume_aligned(xc + 6 * (int64_t)cj[k], 8);
 #pragma unroll
@@ -1057,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_sa_prolong_node(
     __builtin_amdgcn_sched_barrier(0);  // every gather issued before the first product
 #pragma unroll
     for (int k = 0; k < kPass; ++k) {
-      const bool ok = p + k < p1;
+      const bool ok = p + 2 * k < p1;
 #pragma unroll
       for (int a = 0; a < BS; ++a)
 #pragma unroll
@@ -1065,7 +1072,13 @@ __global__ __launch_bounds__(256) void k_sa_prolong_node(
     }
   }
 #pragma unroll
-  for (int a = 0; a < BS; ++a) xf[BS * i + a] = x0[a] + acc[a];
+  for (int a = 0; a < BS; ++a) {
+    const KT other = __shfl_xor(acc[a], 1, 64);
+    acc[a] = h == 0 ? acc[a] + other : acc[a];
+  }
+  if (h == 0)
+#pragma unroll
+    for (int a = 0; a < BS; ++a) xf[BS * i + a] = x0[a] + acc[a];
 }
 
 // x = Ainv b on the coarsest level (dense): one wave per row, lanes over its
@@ -1609,8 +1622,8 @@ int vcycle_t(Ctx* c, const KT* b0, KT* z0, const PcgState* st, bool d0_ready, hi
     // there out of place
     KT* z = l == 0 ? z0 : (KT*)L.d_x;
     if (l == 0)
-      hipLaunchKernelGGL((k_sa_prolong_node<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
-                         dim3(nblocks8(L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
+      hipLaunchKernelGGL((k_sa_prolong_pair<KT, 3, (sizeof(KT) == 4 ? 4 : 2)>),
+                         dim3(nblocks8(2 * L.n, 256)), dim3(256), 0, s, L.d_p_ptr, L.d_p_col,
                          (const PT_t<KT>*)L.d_P, (const KT*)C.d_x, x, L.n, st);
     else
       hipLaunchKernelGGL((k_sa_prolong<KT, 6>), dim3(nblocks(6 * L.n, 256)), dim3(256), 0, s,
