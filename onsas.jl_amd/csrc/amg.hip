@@ -57,7 +57,7 @@ constexpr int kPowerWarm = 3;        // ... re-estimates from the last iterate
 constexpr double kProlongWeight = 1.15;
 constexpr int64_t kSell6MinRows = 2048;  // smaller levels: one wave per block row
 constexpr int kSell6Window = 256;     // rows sorted by length within windows of this size
-constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 4 waves restrict
+constexpr int kRestrictWide = 96;     // P blocks per aggregate above which 2 waves restrict
 // LDS of a sliced-ELL coarse product's workgroup for the x values of its column
 // window (2,048 nodes in FP32, 1,024 in FP64)
 constexpr size_t kX6Lds = 48 * 1024;
@@ -879,8 +879,11 @@ __global__ void k_jacobi(const KT* __restrict__ Dinv, const KT* __restrict__ v, 
 // b_c[I] = sum over the P blocks of column I (entries rt_ptr[I].., row order; PT
 // holds them contiguously) of P_e^T r_row(e), LPA lanes per aggregate over its
 // entries: 32 (two aggregates per wave, half the waves: the fine level, about
-// a hundred entries each), 64, or 256 (the workgroup's four waves add their
-// sums in wave order through LDS).  With d0 non-null also d0[I] = s D_c^-1
+// a hundred entries each), 64, 128 (two waves an aggregate, two aggregates a
+// workgroup: the levels with a few hundred entries an aggregate; on the 64^3
+// cube's level 1 14.3 us against 17.4 us with 256) or 256 (four waves); the
+// waves of an aggregate add their sums in wave order through LDS.  With d0
+// non-null also d0[I] = s D_c^-1
 // b_c[I] (the first Chebyshev step of the coarse level).  An FP32 lane takes
 // two of its entries per pass (LPA apart) with clamped, unconditional index
 // loads, so both row gathers are in flight together (FP64: one, for the
@@ -891,15 +894,20 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
     const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_row,
     const PT_t<KT>* __restrict__ PT, const KT* __restrict__ rf, KT* __restrict__ bc,
     const KT* __restrict__ Dinv_c, KT s, KT* __restrict__ d0, int64_t nc, const PcgState* st) {
-  static_assert(LPA == 32 || LPA == 64 || LPA == 256, "32, 64 or 256 lanes per aggregate");
+  static_assert(LPA == 32 || LPA == 64 || LPA == 128 || LPA == 256,
+                "32, 64, 128 or 256 lanes per aggregate");
   using PV = PT_t<KT>;
   constexpr int kU = sizeof(KT) == 4 ? 2 : 1;
   __shared__ KT sh[4][6];
   // LPA < 256: an XCD-aware run of aggregates (the grid is a multiple of 8)
   const int64_t bl = LPA < 256 ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
-  const int64_t I = (bl * 256 + threadIdx.x) / LPA;
+  const int64_t Iw = (bl * 256 + threadIdx.x) / LPA;
   const int l = threadIdx.x % LPA;
-  if (LPA < 256 && I >= nc) return;  // LPA 256: one aggregate per workgroup, I < nc
+  if (LPA < 128 && Iw >= nc) return;  // no barrier below for these
+  // LPA 128: both aggregates of the workgroup reach the barrier (one past the
+  // end reads the last aggregate and writes nothing); LPA 256: Iw < nc
+  const bool live = Iw < nc;
+  const int64_t I = live ? Iw : nc - 1;
   // the coarse row's D^-1 first (its latency hides under the gathers); an
   // aggregate index uniform over the wave makes its bounds scalar loads
   KT dm[6];
@@ -951,15 +959,21 @@ __global__ __launch_bounds__(256) void k_sa_restrict(
   for (int o = (LPA < 64 ? LPA : 64) / 2; o > 0; o >>= 1)
 #pragma unroll
     for (int q = 0; q < 6; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
-  if (LPA == 256) {
+  if (LPA >= 128) {  // the aggregate's waves add their sums in wave order
     if ((threadIdx.x & 63) < 6) sh[threadIdx.x >> 6][threadIdx.x & 63] = pick6(acc, threadIdx.x & 63);
     __syncthreads();
     if (l < 6) {
+      const int w0 = (int)(threadIdx.x >> 6) & ~(LPA / 64 - 1);  // the aggregate's first wave
 #pragma unroll
-      for (int u = 0; u < 6; ++u) acc[u] = sh[0][u] + sh[1][u] + sh[2][u] + sh[3][u];
+      for (int u = 0; u < 6; ++u) {
+        KT a = sh[w0][u];
+#pragma unroll
+        for (int w = 1; w < LPA / 64; ++w) a += sh[w0 + w][u];
+        acc[u] = a;
+      }
     }
   }
-  if (l >= 6) return;
+  if (l >= 6 || !live) return;
   bc[6 * I + l] = pick6(acc, l);
   if (d0) {
     KT dr = 0;
@@ -1553,7 +1567,7 @@ template <typename KT, int BS>
 void launch_restrict(Ctx* c, const AmgLevel& L, const AmgLevel& C, const KT* r, KT sc, KT* d0c,
                      const PcgState* st) {
   if (L.restrict_wide)
-    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 256>), dim3((unsigned)C.n), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((k_sa_restrict<KT, BS, 128>), dim3(nblocks8(C.n, 2)), dim3(256), 0, c->stream,
                        L.d_rt_ptr, L.d_rt_row, (const PT_t<KT>*)L.d_PT, r, (KT*)C.d_b,
                        (const KT*)C.d_Dinv, sc, d0c, C.n, st);
   else

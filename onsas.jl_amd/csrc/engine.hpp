@@ -151,7 +151,7 @@ struct AmgLevel {
   int32_t* d_agg = nullptr;           // node -> aggregate
   void* d_T = nullptr;                // tentative prolongator, bs x 6 per node (KT)
   int64_t nnz_p = 0;                  // blocks of P
-  bool restrict_wide = false;         // restriction with 4 waves per aggregate
+  bool restrict_wide = false;         // restriction with 2 waves per aggregate
   int32_t *d_p_ptr = nullptr, *d_p_col = nullptr, *d_p_row = nullptr;
   void* d_P = nullptr;                // nnz_p x bs x 6 (KT), row-major blocks
   int64_t* d_pk_ptr = nullptr;        // per P block: the operator blocks of its row
