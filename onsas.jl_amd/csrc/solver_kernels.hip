@@ -394,7 +394,6 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_last = rr;
   const KT bK = (KT)beta;
-  const bool first = (k == 0);
   int64_t r0, r1;
   wg_rows(np, chunk, nn, r0, r1);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -404,6 +403,19 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
     const int32_t b0 = sell_ptr[s];
     const int32_t rounds = slice_rounds[s];
     const int64_t i = kSellRows * s + lane % kSellRows;
+    const bool own = lane < kSellRows && i < r1;
+    // the row's own operands first (independent of the product): z, and for
+    // k > 0 p_old and q_old
+    const int64_t ic = own ? i : r0;
+    KT zi[3], pi[3] = {0, 0, 0}, qi[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) zi[c] = z[3 * ic + c];
+    if (k > 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pi[c] = pold[3 * ic + c];
+        qi[c] = q[3 * ic + c];
+      }
     const int32_t* cp = scol + b0 + lane;
     const KT* vp = sval + 9u * (uint32_t)b0 + lane;
     const uint32_t rs = (uint32_t)rstride;
@@ -412,12 +424,7 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
     for (int m = 0; m < rounds; ++m) {
       const uint32_t j = 3u * (uint32_t)cp[rs * m];
       const KT* v = vp + 9u * rs * m;
-      KT p0 = z[j], p1 = z[j + 1], p2 = z[j + 2];
-      if (!first) {
-        p0 += bK * pold[j];
-        p1 += bK * pold[j + 1];
-        p2 += bK * pold[j + 2];
-      }
+      const KT p0 = z[j], p1 = z[j + 1], p2 = z[j + 2];
       a0 += v[0] * p0 + v[64] * p1 + v[128] * p2;
       a1 += v[192] * p0 + v[256] * p1 + v[320] * p2;
       a2 += v[384] * p0 + v[448] * p1 + v[512] * p2;
@@ -428,20 +435,23 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
       a1 += __shfl_xor(a1, o, 64);
       a2 += __shfl_xor(a2, o, 64);
     }
-    if (lane < kSellRows && i < r1) {
-      KT p0 = z[3 * i], p1 = z[3 * i + 1], p2 = z[3 * i + 2];
-      if (k > 0) {
-        p0 += bK * pold[3 * i];
-        p1 += bK * pold[3 * i + 1];
-        p2 += bK * pold[3 * i + 2];
+    if (own) {
+      // p = z + beta p_old and q = K p = K z + beta q_old (q_old = K p_old
+      // from the previous iteration, the recurrence of Chronopoulos and Gear),
+      // so only z is gathered
+      KT p[3] = {zi[0], zi[1], zi[2]}, a[3] = {a0, a1, a2};
+      if (k > 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          p[c] += bK * pi[c];
+          a[c] += bK * qi[c];
+        }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pnew[3 * i + c] = p[c];
+        q[3 * i + c] = a[c];
       }
-      pnew[3 * i] = p0;
-      pnew[3 * i + 1] = p1;
-      pnew[3 * i + 2] = p2;
-      q[3 * i] = a0;
-      q[3 * i + 1] = a1;
-      q[3 * i + 2] = a2;
-      pq += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
+      pq += (double)p[0] * (double)a[0] + (double)p[1] * (double)a[1] + (double)p[2] * (double)a[2];
     }
   }
   pq = block_sum<kSpmvThreads>(pq, sh);
