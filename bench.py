@@ -139,12 +139,13 @@ def run_mode(s, sm, p, precision, steps, warmup, nsteps, forcing=False,
 def asm_rates(r, kt, ne):
     """Assembly throughput and its HBM roofline fraction.  Gather assembly
     (NeoHookean: 24 state planes), every datum moved once:
-      element pass: conn 16 + dN/V 104 per tet, U 24 per node, state write
-                    24 kt + nodal forces 96 per tet
+      element pass: conn 16 per tet, U and the reference coordinates 24 + 24
+                    per node (the gradients are recomputed from them), state
+                    write 24 kt + nodal forces 96 per tet
       gather pass : contribution entries 16 x 4, state read 24 kt, forces 96
                     per tet; K 9 kt per block written once; row_ptr/col;
                     fixed mask, block-Jacobi 9 kt and f_int 24 per node.
-                    Staged (the default): the stored 16-bit entries (with
+                    Staged (ONSAS_GATHER=staged): the stored 16-bit entries (with
                     padding) instead of 16 x 4, the element lists 4 per staged
                     element, the step table 16 per step and 24 per workgroup,
                     the diagonal index 4 per node; the states once.
@@ -157,7 +158,7 @@ def asm_rates(r, kt, ne):
     asm_ms = tm["assemble_ms"] / na
     elem_ms = tm["elem_ms"] / na
     gather_ms = (tm["scatter_ms"] - tm["elem_ms"]) / na
-    b_elem = ne * (16 + 104 + 24 * kt + 96) + 24 * nn
+    b_elem = ne * (16 + 24 * kt + 96) + 48 * nn
     b_gather = (ne * (64 + 24 * kt + 96) + 9 * kt * nnzb + 4 * (nn + 1) + 4 * nnzb +
                 nn * (1 + 9 * kt + 24))
     lay = r.get("layout") or {}

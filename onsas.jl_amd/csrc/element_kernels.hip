@@ -430,16 +430,69 @@ constexpr bool tet_plane_ok() {
 }
 static_assert(tet_plane_ok(), "tet state layout");
 
+// Shape-function gradients and volume of a tet from its reference coordinates
+// X (node-major): the operations of the set-up's tet_gradients (setup.hip),
+// which fills the stored geometry the other element kernels read, in the same
+// order and without contraction: J = X^T D, V = det J / 6, dN = D^T J^-T.
+__device__ inline void tet_gradients_dev(const double X[12], double n[4][3], double& V) {
+#pragma clang fp contract(off)
+  constexpr double D[3][4] = {{1, -1, 0, 0}, {0, -1, 0, 1}, {0, -1, 1, 0}};
+  double J[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double s = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) s += X[b * 3 + i] * D[k][b];
+      J[i * 3 + k] = s;
+    }
+  const double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                     J[2] * (J[3] * J[7] - J[4] * J[6]);
+  V = det / 6.0;
+  const double Ji[9] = {(J[4] * J[8] - J[5] * J[7]) / det, (J[2] * J[7] - J[1] * J[8]) / det,
+                        (J[1] * J[5] - J[2] * J[4]) / det, (J[5] * J[6] - J[3] * J[8]) / det,
+                        (J[0] * J[8] - J[2] * J[6]) / det, (J[2] * J[3] - J[0] * J[5]) / det,
+                        (J[3] * J[7] - J[4] * J[6]) / det, (J[1] * J[6] - J[0] * J[7]) / det,
+                        (J[0] * J[4] - J[1] * J[3]) / det};
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += Ji[k * 3 + j] * D[k][b];
+      n[b][j] = s;
+    }
+}
+
+// Element pass of the gather assembly: the tet's gradients are recomputed
+// from the nodal reference coordinates (read with U at the same nodes, mostly
+// from L2) instead of streaming the stored 104-byte geometry record.
 template <typename KT>
 __global__ __launch_bounds__(256) void k_tet_state(TetArgs ta, const double* __restrict__ U,
+                                                   const double* __restrict__ Xr,
+                                                   const double* __restrict__ Yr,
+                                                   const double* __restrict__ Zr,
                                                    KT* __restrict__ st, double* __restrict__ fe,
                                                    int hasM) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= ta.ne) return;
   const int64_t ne = ta.ne;
   int32_t nd[4];
-  double n[4][3], u[4][3], V, p1;
-  tet_load(ta, e, U, nd, n, V, u, p1);
+  double n[4][3], u[4][3], V, X[12];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) nd[b] = ta.conn[b * ne + e];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    X[b * 3] = Xr[nd[b]];
+    X[b * 3 + 1] = Yr[nd[b]];
+    X[b * 3 + 2] = Zr[nd[b]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[b][d] = U[3 * (int64_t)nd[b] + d];
+  }
+  const double p1 = ta.G ? ta.G[e] : ta.p1;
+  tet_gradients_dev(X, n, V);
   TetEval ev;
   tet_eval(ta.mat, ta.p0, p1, n, u, ev);
 #pragma unroll
@@ -1545,7 +1598,8 @@ int assemble_gather_t(Ctx* c) {
     const ElemGroup& g = c->groups[gi];
     if (g.type == kTet)
       hipLaunchKernelGGL(k_tet_state<KT>, dim3(grid_for(g.ne, 256)), dim3(256), 0, s,
-                         tet_args(g, 0, g.ne), c->d_U, (KT*)g.d_state, g.d_fe, (int)g.hasM);
+                         tet_args(g, 0, g.ne), c->d_U, c->d_x, c->d_y, c->d_z, (KT*)g.d_state,
+                         g.d_fe, (int)g.hasM);
     else
       hipLaunchKernelGGL(k_truss_state<KT>, dim3(grid_for(g.ne, 256)), dim3(256), 0, s,
                          truss_args(g, 0, g.ne), c->d_U, (KT*)g.d_state, g.d_fe);
