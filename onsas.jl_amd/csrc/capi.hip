@@ -306,7 +306,7 @@ void onsas_destroy(onsas_ctx* ctx) {
                   c->d_gm_h, c->d_sell_ptr, c->d_sell_col, c->d_Ksell, c->d_slice_rounds,
                   c->d_contrib, c->d_sell_groups, (void*)c->d_stage_groups,
                   (void*)c->d_stage_steps, c->d_stage, c->d_stage_words, c->d_stage_pairs,
-                  c->d_pair_row, c->d_colmask, c->d_snap[0], c->d_snap[1]};
+                  c->d_pair_row, c->d_colmask, c->d_snap[0], c->d_snap[1], c->d_ppq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_sc) (void)hipHostFree(c->h_sc);

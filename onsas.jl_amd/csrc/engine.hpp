@@ -296,6 +296,7 @@ struct Ctx {
   void *d_px = nullptr, *d_pr = nullptr, *d_pz = nullptr, *d_pp0 = nullptr, *d_pp1 = nullptr,
        *d_pq = nullptr;  // PCG vectors (KT)
   double* d_part = nullptr;  // partial-sum arrays, kPartArrays x kMaxPartials
+  double* d_ppq = nullptr;   // p.q partials of k_pcg_A, one per workgroup of 4 slices
   Scalars* d_sc = nullptr;
   Scalars* h_sc = nullptr;   // pinned
   double* h_stage = nullptr;  // pinned, 3 nn values: staging of state vector copies

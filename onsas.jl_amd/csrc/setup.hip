@@ -441,6 +441,7 @@ int finalize_model(Ctx* c) {
   // sliced-ELL layout of the tangent (kSellRows block rows per slice, padded
   // with zero blocks whose column is the row itself)
   c->nslices = (nn + kSellRows - 1) / kSellRows;
+  ONSAS_TRY(dev_alloc(c, (void**)&c->d_ppq, sizeof(double) * (size_t)((c->nslices + 3) / 4)));
   c->sell_ptr.assign(c->nslices + 1, 0);
   c->slice_rounds.assign(c->nslices, 0);
   c->ell_width = 0;

@@ -354,23 +354,22 @@ __global__ __launch_bounds__(256) void k_unpack_sell(const int32_t* __restrict__
 }
 
 // pcg_A on the sliced-ELL tangent: one wave per slice of kSellRows block rows,
-// kSellLanes lanes (slot phases) per row combined by one shuffle stage.
-// Workgroup w owns rows [r0, r0 + chunk) (whole slices); its waves stride over
-// those slices.
+// kSellLanes lanes (slot phases) per row combined by one shuffle stage, four
+// slices per workgroup (the cycle's fine-level product's layout); one p.q
+// partial per workgroup.
 template <typename KT>
-__global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A(const int32_t* __restrict__ sell_ptr,
-                                                const int32_t* __restrict__ slice_rounds,
-                                                int64_t rstride,
-                                                const int32_t* __restrict__ scol,
-                                                const KT* __restrict__ sval, const KT* __restrict__ z,
-                                                   const KT* __restrict__ pold, KT* __restrict__ pnew,
-                                                   KT* __restrict__ q, const double* __restrict__ prz_cur,
-                                                   const double* __restrict__ prz_prev,
-                                                   const double* __restrict__ prr,
-                                                   double* __restrict__ ppq, PcgState* __restrict__ st,
-                                                   int np, int nrz, int64_t chunk, int64_t nn,
-                                                   int par) {
-  __shared__ double sh[3 * kSpmvThreads / 64];
+__global__ __launch_bounds__(256) void k_pcg_A(const int32_t* __restrict__ sell_ptr,
+                                               const int32_t* __restrict__ slice_rounds,
+                                               int64_t rstride, const int32_t* __restrict__ scol,
+                                               const KT* __restrict__ sval, const KT* __restrict__ z,
+                                               const KT* __restrict__ pold, KT* __restrict__ pnew,
+                                               KT* __restrict__ q, const double* __restrict__ prz_cur,
+                                               const double* __restrict__ prz_prev,
+                                               const double* __restrict__ prr,
+                                               double* __restrict__ ppq, PcgState* __restrict__ st,
+                                               int np, int nrz, int64_t nslices, int64_t nn,
+                                               int par) {
+  __shared__ double sh[3 * 256 / 64];
   if (st->done | st->breakdown) return;
   const int k = st->k;
   // r.z of this iteration from its partials; that of the previous one as the
@@ -378,7 +377,7 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
   // being written)
   double rz, rr, unused;
   (void)prz_prev;
-  sum_parts3<kSpmvThreads>(prz_cur, prr, nullptr, nrz, np, sh, rz, rr, unused);
+  sum_parts3<256>(prz_cur, prr, nullptr, nrz, np, sh, rz, rr, unused);
   const double rzp = k > 0 ? st->rzs[par ^ 1] : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rzs[par] = rz;
   const double beta = k > 0 ? rz / rzp : 0.0;
@@ -394,19 +393,17 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rr_last = rr;
   const KT bK = (KT)beta;
-  int64_t r0, r1;
-  wg_rows(np, chunk, nn, r0, r1);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t s_end = (r1 + kSellRows - 1) / kSellRows;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   double pq = 0;
-  for (int64_t s = r0 / kSellRows + wave; s < s_end; s += kSpmvThreads / 64) {
+  if (s < nslices) {
     const int32_t b0 = sell_ptr[s];
     const int32_t rounds = slice_rounds[s];
     const int64_t i = kSellRows * s + lane % kSellRows;
-    const bool own = lane < kSellRows && i < r1;
+    const bool own = lane < kSellRows && i < nn;
     // the row's own operands first (independent of the product): z, and for
     // k > 0 p_old and q_old
-    const int64_t ic = own ? i : r0;
+    const int64_t ic = own ? i : kSellRows * s;
     KT zi[3], pi[3] = {0, 0, 0}, qi[3] = {0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; ++c) zi[c] = z[3 * ic + c];
@@ -454,7 +451,7 @@ __global__ __launch_bounds__(kSpmvThreads, ONSAS_SPMV_WAVES_PER_EU) void k_pcg_A
       pq += (double)p[0] * (double)a[0] + (double)p[1] * (double)a[1] + (double)p[2] * (double)a[2];
     }
   }
-  pq = block_sum<kSpmvThreads>(pq, sh);
+  pq = block_sum<256>(pq, sh);
   if (threadIdx.x == 0) ppq[blockIdx.x] = pq;
 }
 
@@ -466,11 +463,11 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B(const KT* __restrict__ p, const
                                                    const double* __restrict__ ppq,
                                                    double* __restrict__ prz_next,
                                                    double* __restrict__ prr, PcgState* __restrict__ st,
-                                                   int np, int64_t chunk, int64_t nn) {
+                                                   int np, int npq, int64_t chunk, int64_t nn) {
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
   double rz, pq, unused;
-  sum_parts3(prz_cur, ppq, nullptr, np, np, sh, rz, pq, unused);
+  sum_parts3(prz_cur, ppq, nullptr, np, npq, sh, rz, pq, unused);
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
@@ -643,15 +640,14 @@ __global__ __launch_bounds__(NT, 8) void k_pcg_B_r(const KT* __restrict__ p, con
                                                      const double* __restrict__ ppq,
                                                      double* __restrict__ prr, PcgState* __restrict__ st,
                                                      const KT* __restrict__ Dinv, KT s,
-                                                     KT* __restrict__ d0, int np, int nrz,
+                                                     KT* __restrict__ d0, int np, int npq,
                                                      int64_t chunk, int64_t nn, int par) {
   __shared__ double sh[3 * NT / 64];
   if (st->done | st->breakdown) return;
   // r.z as k_pcg_A summed it (same value bit for bit); p.q from its partials
   double pq, unused0, unused2;
   (void)prz_cur;
-  (void)nrz;
-  sum_parts3(ppq, ppq, nullptr, 0, np, sh, unused0, pq, unused2);
+  sum_parts3(ppq, ppq, nullptr, 0, npq, sh, unused0, pq, unused2);
   const double rz = st->rzs[par];
   if (!(pq > 0.0) || !isfinite(pq) || !isfinite(rz)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -907,24 +903,24 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
   const int nrz = rz_count(c);
   PcgState* st = &c->d_sc->pcg;
   const int64_t rstride = c->ell_rounds ? 64 * c->nslices : 64;
+  const int npq = (int)((c->nslices + 3) / 4);  // k_pcg_A's workgroups
   if (ea)  // sampled launch: the dispatch itself stamps the start and stop events
-    hipExtLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, ea, eb, 0,
-                          c->d_sell_ptr, c->d_slice_rounds, rstride, c->d_sell_col,
-                          (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par],
-                          (KT*)c->d_pq, RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ),
-                          st, c->np, nrz, c->chunk, c->nn, par);
+    hipExtLaunchKernelGGL(k_pcg_A<KT>, dim3(npq), dim3(256), 0, s, ea, eb, 0, c->d_sell_ptr,
+                          c->d_slice_rounds, rstride, c->d_sell_col, (const KT*)c->d_Ksell,
+                          (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq, RZ[par],
+                          RZ[1 - par], part(c, kPartRR), c->d_ppq, st, c->np, nrz, c->nslices,
+                          c->nn, par);
   else
-    hipLaunchKernelGGL(k_pcg_A<KT>, dim3(c->np), dim3(kSpmvThreads), 0, s, c->d_sell_ptr,
-                       c->d_slice_rounds, rstride, c->d_sell_col,
-                       (const KT*)c->d_Ksell, (const KT*)c->d_pz, P[1 - par], P[par], (KT*)c->d_pq,
-                       RZ[par], RZ[1 - par], part(c, kPartRR), part(c, kPartPQ), st, c->np, nrz,
-                       c->chunk, c->nn, par);
+    hipLaunchKernelGGL(k_pcg_A<KT>, dim3(npq), dim3(256), 0, s, c->d_sell_ptr, c->d_slice_rounds,
+                       rstride, c->d_sell_col, (const KT*)c->d_Ksell, (const KT*)c->d_pz,
+                       P[1 - par], P[par], (KT*)c->d_pq, RZ[par], RZ[1 - par], part(c, kPartRR),
+                       c->d_ppq, st, c->np, nrz, c->nslices, c->nn, par);
   if (c->lin.precond == ONSAS_PRECOND_AMG) {
     KT* d0 = nullptr;
     const double s0 = amg_fine_smoother(c, (void**)&d0);
     hipLaunchKernelGGL(k_pcg_B_r<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
-                       (KT*)c->d_px, (KT*)c->d_pr, RZ[par], part(c, kPartPQ), part(c, kPartRR), st,
-                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, nrz, c->chunk, c->nn, par);
+                       (KT*)c->d_px, (KT*)c->d_pr, RZ[par], c->d_ppq, part(c, kPartRR), st,
+                       (const KT*)c->d_Dinv, (KT)s0, d0, c->np, npq, c->chunk, c->nn, par);
     // the stop check (||r_{k+1}||^2 from the update's partials) runs in the
     // prologue of the cycle's first tangent product
     const bool fused = rz_from_cycle(c);
@@ -937,7 +933,7 @@ void launch_pcg_iteration(Ctx* c, int par, hipEvent_t ea, hipEvent_t eb, hipEven
   }
   hipLaunchKernelGGL(k_pcg_B<KT>, dim3(c->np), dim3(NT), 0, s, P[par], (const KT*)c->d_pq,
                      (const KT*)c->d_Dinv, (KT*)c->d_px, (KT*)c->d_pr, (KT*)c->d_pz, RZ[par],
-                     part(c, kPartPQ), RZ[1 - par], part(c, kPartRR), st, c->np, c->chunk, c->nn);
+                     c->d_ppq, RZ[1 - par], part(c, kPartRR), st, c->np, npq, c->chunk, c->nn);
 }
 
 // Graph slot `slot` holding `iters` PCG iterations (even: parity 0 to 0), captured
