@@ -107,6 +107,43 @@ __device__ inline void sum_parts3(const double* __restrict__ p0, const double* _
   }
 }
 
+// Sums of two partial arrays (p0 of n0 entries, p1 of n1) by a 256-thread
+// workgroup in the order of sum_parts3<512>, which the 512-thread reductions
+// and the multigrid cycle's stop check apply: virtual thread t' adds entries
+// t', t' + 512, ... in order, then the butterfly over its wave, then the 8
+// wave sums in order; threads t and t + 256 are one thread's two accumulators.
+// sh needs 16 doubles.
+__device__ inline void sum_parts2_512(const double* __restrict__ p0, const double* __restrict__ p1,
+                                      int n0, int n1, double* sh, double& s0, double& s1) {
+  const int t = threadIdx.x;
+  double a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+  for (int i = t; i < n0; i += 512) a0 += p0[i];
+  for (int i = t + 256; i < n0; i += 512) b0 += p0[i];
+  for (int i = t; i < n1; i += 512) a1 += p1[i];
+  for (int i = t + 256; i < n1; i += 512) b1 += p1[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, 64);
+    b0 += __shfl_xor(b0, o, 64);
+    a1 += __shfl_xor(a1, o, 64);
+    b1 += __shfl_xor(b1, o, 64);
+  }
+  __syncthreads();
+  if ((t & 63) == 0) {
+    sh[t >> 6] = a0;
+    sh[4 + (t >> 6)] = b0;
+    sh[8 + (t >> 6)] = a1;
+    sh[12 + (t >> 6)] = b1;
+  }
+  __syncthreads();
+  s0 = s1 = 0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) {
+    s0 += sh[w];
+    s1 += sh[8 + w];
+  }
+}
+
 // XCD-aware row range: consecutive hardware workgroup ids land on different
 // XCDs (round-robin), so logical chunk = (id % 8) * (np / 8) + id / 8 keeps
 // each XCD on a contiguous band of rows (its L2 sees the neighbouring columns).
@@ -369,15 +406,16 @@ __global__ __launch_bounds__(256) void k_pcg_A(const int32_t* __restrict__ sell_
                                                double* __restrict__ ppq, PcgState* __restrict__ st,
                                                int np, int nrz, int64_t nslices, int64_t nn,
                                                int par) {
-  __shared__ double sh[3 * 256 / 64];
+  __shared__ double sh[16];
   if (st->done | st->breakdown) return;
   const int k = st->k;
   // r.z of this iteration from its partials; that of the previous one as the
   // previous launch summed it (kept by parity, so no workgroup reads the slot
   // being written)
-  double rz, rr, unused;
+  // (the 512-wide order: rr as the cycle's stop check sums it)
+  double rz, rr;
   (void)prz_prev;
-  sum_parts3<256>(prz_cur, prr, nullptr, nrz, np, sh, rz, rr, unused);
+  sum_parts2_512(prz_cur, prr, nrz, np, sh, rz, rr);
   const double rzp = k > 0 ? st->rzs[par ^ 1] : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->rzs[par] = rz;
   const double beta = k > 0 ? rz / rzp : 0.0;
