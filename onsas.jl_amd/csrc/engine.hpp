@@ -32,16 +32,13 @@ constexpr int kRowGroup = 16;      // lanes per block row in the block-CSR SpMV 
 #ifndef ONSAS_SPMV_THREADS
 #define ONSAS_SPMV_THREADS 512
 #endif
-#ifndef ONSAS_SPMV_WAVES_PER_EU
-#define ONSAS_SPMV_WAVES_PER_EU 1
-#endif
 #ifndef ONSAS_SPMV_UNROLL
 #define ONSAS_SPMV_UNROLL 1
 #endif
 constexpr int kSellLanes = ONSAS_SELL_LANES;     // lanes per block row in the sliced-ELL SpMV
 constexpr int kSellRows = 64 / kSellLanes;       // block rows per slice (one wave per slice)
-constexpr int kSpmvThreads = ONSAS_SPMV_THREADS; // threads per workgroup of the sliced-ELL SpMV
-constexpr int kChunkRows = (kSpmvThreads / 64) * kSellRows;  // rows per workgroup chunk unit
+constexpr int kSpmvThreads = ONSAS_SPMV_THREADS; // a reduction workgroup's waves x 64
+constexpr int kChunkRows = (kSpmvThreads / 64) * kSellRows;  // rows per workgroup chunk unit (whole slices)
 constexpr int kMaxPartials = 1024; // upper bound of the reduction grid (multiple of 8)
 
 enum GroupType { kTet = 0, kTruss = 1 };
